@@ -1,0 +1,92 @@
+/* libssn.so — MI355X (gfx950) Dual solver for the nearest real symplectic Stiefel matrix:
+ *   min_X ||X - T||_F^2  s.t.  X^T J_2n X = J_2k,   T of shape 2n x 2k (m = 2k), FP64.
+ *
+ * C ABI: plain pointers and sizes. Every pointer argument except `stream` is a DEVICE
+ * pointer owned by the caller. Every call is asynchronous on `stream` (a hipStream_t,
+ * or NULL for the default stream); nothing is allocated inside the library: scratch
+ * comes in through `ws` / `ws_bytes`, sized by ssn_workspace_bytes(). Functions return
+ * SSN_OK (0) or a negative SSN_ERR_* code and never throw. Per-target solver outcomes
+ * are written to the device array `status` (SSN_ST_*).
+ *
+ * Matrices are row-major with element stride 1: a batch of matrices is addressed as
+ * base + b * batch_stride + row * row_stride + col. "mp" below is ssn_padded_order(m)
+ * = 16 * ceil(m / 16); [batch][mp][mp] arrays are dense and zero padded.
+ *
+ * Supported shapes in this build: n >= 1, m even, 2 <= m <= 128.
+ */
+#ifndef SSN_H
+#define SSN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSN_OK 0
+#define SSN_ERR_ARGS (-1)        /* bad shape / null pointer / leading dimension        */
+#define SSN_ERR_HIP (-2)         /* a HIP call failed; see ssn_last_hip_error()         */
+#define SSN_ERR_WORKSPACE (-3)   /* ws missing or smaller than ssn_workspace_bytes()    */
+#define SSN_ERR_UNSUPPORTED (-4) /* shape not supported on this device (e.g. LDS size)  */
+#define SSN_ERR_INIT (-5)        /* no current HIP device                               */
+
+/* per-target status values written by ssn_dual / ssn_solve */
+#define SSN_ST_OK 0
+#define SSN_ST_NO_DESCENT 1      /* Armijo backtracking exhausted (dual_recommended_hvp.py:189-190) */
+#define SSN_ST_NEWTON_CAP 2      /* MAX_NEWTON = 100 reached (dual_recommended_hvp.py:191)          */
+#define SSN_ST_NONFINITE 3       /* non-finite residual                                             */
+#define SSN_ST_EIGH 4            /* Jacobi eigensolver did not converge                             */
+
+/* Replaces initialize_kernels() (src/symplectic_projection/__init__.py:10-19): there is
+ * nothing to JIT-compile; it caches the current device's LDS limit. Idempotent. */
+int ssn_init(void);
+
+const char* ssn_version(void);
+int ssn_padded_order(int m);
+int ssn_last_hip_error(void);
+const char* ssn_status_string(int status);
+
+/* Bytes of device scratch needed by ssn_solve (and sufficient for ssn_gram, ssn_dual,
+ * ssn_recover) for `batch` targets of shape 2n x m. 0 for unsupported shapes. */
+size_t ssn_workspace_bytes(int batch, int n, int m);
+
+/* Cr = A^T A + D^T D, Ci = A^T D - D^T A for T = [A; D] ([batch][2n][m], strides sT_b, sT_r).
+ * Cr, Ci: [batch][m][m] with strides sC_b, ldc.
+ * Replaces gram_add_ / gram_offdiag / gram_diagonal and the chunk loop of gram_chunks
+ * (gram_triangle_reflect.py:19-36, gram_triangle_reflect.cu:1-104, dual_recommended_hvp.py:79-96). */
+int ssn_gram(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+             double* Cr, double* Ci, long long sC_b, int ldc, void* ws, size_t ws_bytes);
+
+/* Dual Newton / PCG / Armijo from the Gram parts, entirely on device, one workgroup per target.
+ * Outputs: omega [batch][m][m] (strides sO_b, ldo) in the original basis; P, Q [batch][mp][mp],
+ * the inverse parts (I + i omega)^-1 = P + iQ used by recovery; iters [batch] Newton updates;
+ * status [batch]; counters [batch][4] = newton, cg iterations, backtracks, HVPs (may be NULL).
+ * Replaces dual_batch (dual_recommended_hvp.py:99-191) with hvp_inner / hvp_outer /
+ * hvp_outer_off16 / hvp_outer_diag16 / hvp_sandwich_front / hvp_line_partial / hvp_line_finish /
+ * hvp_precondition / hvp_cg_update / hvp_cg_next (hvp_fused.cu:1-420, hvp_fused.py:22-143),
+ * triangular_inverse / inverse_product (cholesky_inverse_kernel.cu:1-103) and the torch.linalg
+ * eigh / cholesky_ex / cholesky_solve calls on that path. */
+int ssn_dual(void* stream, const double* Cr, const double* Ci, long long sC_b, int ldc, int batch, int m,
+             double* omega, long long sO_b, int ldo, double* P, double* Q,
+             long long* iters, int* status, int* counters, void* ws, size_t ws_bytes);
+
+/* X = [A P^T + D Q^T ; D P^T - A Q^T] with P, Q [batch][mp][mp] from ssn_dual.
+ * X: [batch][2n][m], strides sX_b, sX_r.
+ * Replaces recover_block / recover_fused and recover_matmul
+ * (recovery_fused.py:19-30, recovery_fused.cu:1-41, dual_recommended_hvp.py:194-228). */
+int ssn_recover(void* stream, const double* T, long long sT_b, long long sT_r, const double* P,
+                const double* Q, int batch, int n, int m, double* X, long long sX_b, long long sX_r,
+                void* ws, size_t ws_bytes);
+
+/* Gram -> dual -> recovery in one call (solve, dual_recommended_hvp.py:231-242), device in and
+ * device out. The host-side mirror maps SSN_ST_NO_DESCENT / SSN_ST_NEWTON_CAP to the reference's
+ * batch-wide RuntimeError. */
+int ssn_solve(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+              double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
+              long long* iters, int* status, int* counters, void* ws, size_t ws_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSN_H */

@@ -1,0 +1,16 @@
+"""MI355X-native Dual solver for the nearest real symplectic Stiefel matrix.
+
+Drop-in for the reference package `symplectic_projection` (src/symplectic_projection/__init__.py:1-19):
+
+    import symplectic_stiefel_nearness_amd as symplectic_projection
+    symplectic_projection.initialize_kernels()
+    X, omega, iterations = symplectic_projection.solve(T)   # T: float64 [batch, 2n, 2k]
+
+The arithmetic runs in libssn.so (hand-written HIP for gfx950) through a C ABI
+(include/ssn.h); this package is the host-side mirror of the reference interface.
+"""
+from .dual import (initialize_kernels, solve, solve_device, gram_chunks, dual_batch,
+                   recover_matmul, FEAS_TOL, MAX_NEWTON, MAX_CG, MAX_BACKTRACK)
+
+__all__ = ["initialize_kernels", "solve", "solve_device", "gram_chunks", "dual_batch",
+           "recover_matmul"]

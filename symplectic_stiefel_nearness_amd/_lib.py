@@ -1,0 +1,73 @@
+"""ctypes binding of libssn.so (the C ABI declared in include/ssn.h).
+
+Tensors cross the boundary as raw device pointers plus int64 strides; the stream is
+torch's current HIP stream. No torch types appear in the C signatures.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libssn.so"
+
+SSN_OK = 0
+SSN_ERR_ARGS = -1
+SSN_ERR_HIP = -2
+SSN_ERR_WORKSPACE = -3
+SSN_ERR_UNSUPPORTED = -4
+SSN_ERR_INIT = -5
+
+ST_OK, ST_NO_DESCENT, ST_NEWTON_CAP, ST_NONFINITE, ST_EIGH = 0, 1, 2, 3, 4
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_L = ctypes.c_longlong
+_S = ctypes.c_size_t
+
+# name -> (restype, argtypes); must match include/ssn.h exactly
+SIGNATURES = {
+    "ssn_init": (_I, []),
+    "ssn_version": (ctypes.c_char_p, []),
+    "ssn_padded_order": (_I, [_I]),
+    "ssn_last_hip_error": (_I, []),
+    "ssn_status_string": (ctypes.c_char_p, [_I]),
+    "ssn_workspace_bytes": (_S, [_I, _I, _I]),
+    "ssn_gram": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _P, _L, _I, _P, _S]),
+    "ssn_dual": (_I, [_P, _P, _P, _L, _I, _I, _I, _P, _L, _I, _P, _P, _P, _P, _P, _P, _S]),
+    "ssn_recover": (_I, [_P, _P, _L, _L, _P, _P, _I, _I, _I, _P, _L, _L, _P, _S]),
+    "ssn_solve": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _L, _I, _P, _P, _P, _P, _S]),
+}
+
+_lib = None
+
+
+def load(path: Path | str | None = None):
+    """Load libssn.so once and bind every entry point; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: build it with `python -m symplectic_stiefel_nearness_amd.build` "
+            "(there is no CPU fallback for the solver)")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc == SSN_OK:
+        return
+    lib = load()
+    if rc == SSN_ERR_ARGS:
+        raise AssertionError(f"{what}: invalid arguments")
+    if rc == SSN_ERR_WORKSPACE:
+        raise AssertionError(f"{what}: workspace too small")
+    if rc == SSN_ERR_UNSUPPORTED:
+        raise ValueError(f"{what}: unsupported shape for this device")
+    raise RuntimeError(f"{what}: HIP error {lib.ssn_last_hip_error()} (code {rc})")

@@ -1,0 +1,185 @@
+// extern "C" entry points of libssn.so (declared in include/ssn.h).
+// Plain pointers and sizes only; every device pointer is caller-owned, every call
+// is asynchronous on the caller's stream, and no entry point throws or allocates:
+// scratch comes in through the `ws` argument, sized by ssn_workspace_bytes().
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include "common.h"
+
+namespace ssn {
+int gram_chunk_count(int batch, int n);
+hipError_t launch_gram_partial(hipStream_t, const double*, long long, long long, int, int, int, int, double*);
+hipError_t launch_gram_reduce(hipStream_t, const double*, int, int, int, double*, double*, long long, int);
+hipError_t launch_recovery_fragments(hipStream_t, const double*, const double*, int, int, double*);
+hipError_t launch_recover(hipStream_t, const double*, long long, long long, const double*, int, int, int,
+                          double*, long long, long long);
+size_t dual_workspace_doubles(int batch, int m);
+hipError_t launch_dual(hipStream_t, const double*, const double*, long long, int, int, int,
+                       double*, long long, int, double*, double*, long long*, int*, int*, double*, size_t);
+}  // namespace ssn
+
+using namespace ssn;
+
+#define SSN_OK 0
+#define SSN_ERR_ARGS (-1)
+#define SSN_ERR_HIP (-2)
+#define SSN_ERR_WORKSPACE (-3)
+#define SSN_ERR_UNSUPPORTED (-4)
+#define SSN_ERR_INIT (-5)
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+size_t g_lds_limit[MAX_DEVICES];
+bool g_ready[MAX_DEVICES];
+hipError_t g_last_hip = hipSuccess;
+
+int current_device_ready(size_t* lds) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return SSN_ERR_INIT;
+    if (!g_ready[dev]) {
+        int v = 0;
+        hipError_t e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (e != hipSuccess) { g_last_hip = e; return SSN_ERR_HIP; }
+        g_lds_limit[dev] = (size_t)v;
+        g_ready[dev] = true;
+    }
+    if (lds) *lds = g_lds_limit[dev];
+    return SSN_OK;
+}
+
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return SSN_OK;
+    g_last_hip = e;
+    return e == hipErrorInvalidValue ? SSN_ERR_UNSUPPORTED : SSN_ERR_HIP;
+}
+
+bool shape_ok(int batch, int n, int m) {
+    return batch >= 0 && n >= 1 && m >= 2 && (m % 2) == 0 && m <= 128;
+}
+
+// workspace layout (doubles): [gram partials][Cr][Ci][dual slots][Pf][Qf][recovery fragments]
+struct Layout {
+    size_t partial, cr, ci, dual, pf, qf, frag, total;
+    int chunks;
+};
+
+Layout layout(int batch, int n, int m) {
+    Layout L;
+    const size_t mp = (size_t)pad16(m);
+    L.chunks = gram_chunk_count(batch > 0 ? batch : 1, n);
+    size_t off = 0;
+    L.partial = off; off += (size_t)batch * L.chunks * 2 * mp * mp;
+    L.cr = off;      off += (size_t)batch * m * m;
+    L.ci = off;      off += (size_t)batch * m * m;
+    L.dual = off;    off += dual_workspace_doubles(batch, m);
+    L.pf = off;      off += (size_t)batch * mp * mp;
+    L.qf = off;      off += (size_t)batch * mp * mp;
+    L.frag = off;    off += (size_t)batch * 4 * mp * mp;
+    L.total = off;
+    return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssn_init(void) { return current_device_ready(nullptr); }
+
+const char* ssn_version(void) { return "ssn-gfx950 0.1.0"; }
+
+int ssn_padded_order(int m) { return pad16(m); }
+
+int ssn_last_hip_error(void) { return (int)g_last_hip; }
+
+const char* ssn_status_string(int code) {
+    switch (code) {
+        case ST_OK: return "ok";
+        case ST_NO_DESCENT: return "line search found no descent step";
+        case ST_NEWTON_CAP: return "Newton iteration cap reached";
+        case ST_NONFINITE: return "non-finite residual";
+        case ST_EIGH: return "eigensolver did not converge";
+        default: return "unknown status";
+    }
+}
+
+size_t ssn_workspace_bytes(int batch, int n, int m) {
+    if (!shape_ok(batch, n, m)) return 0;
+    return layout(batch, n, m).total * sizeof(double);
+}
+
+int ssn_gram(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+             double* Cr, double* Ci, long long sC_b, int ldc, void* ws, size_t ws_bytes)
+{
+    if (!shape_ok(batch, n, m) || !T || !Cr || !Ci || ldc < m) return SSN_ERR_ARGS;
+    if (batch == 0) return SSN_OK;
+    int rc = current_device_ready(nullptr);
+    if (rc) return rc;
+    Layout L = layout(batch, n, m);
+    if (!ws || ws_bytes < (L.partial + (size_t)batch * L.chunks * 2 * pad16(m) * pad16(m)) * sizeof(double))
+        return SSN_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    double* part = (double*)ws + L.partial;
+    rc = hip_status(launch_gram_partial(s, T, sT_b, sT_r, batch, n, m, L.chunks, part));
+    if (rc) return rc;
+    return hip_status(launch_gram_reduce(s, part, batch, L.chunks, m, Cr, Ci, sC_b, ldc));
+}
+
+int ssn_dual(void* stream, const double* Cr, const double* Ci, long long sC_b, int ldc, int batch, int m,
+             double* omega, long long sO_b, int ldo, double* P, double* Q,
+             long long* iters, int* status, int* counters, void* ws, size_t ws_bytes)
+{
+    if (!shape_ok(batch, 1, m) || !Cr || !Ci || !omega || !P || !Q || !iters || !status ||
+        ldc < m || ldo < m) return SSN_ERR_ARGS;
+    if (batch == 0) return SSN_OK;
+    size_t lds = 0;
+    int rc = current_device_ready(&lds);
+    if (rc) return rc;
+    if (!ws || ws_bytes < dual_workspace_doubles(batch, m) * sizeof(double)) return SSN_ERR_WORKSPACE;
+    return hip_status(launch_dual((hipStream_t)stream, Cr, Ci, sC_b, ldc, batch, m, omega, sO_b, ldo,
+                                  P, Q, iters, status, counters, (double*)ws, lds));
+}
+
+int ssn_recover(void* stream, const double* T, long long sT_b, long long sT_r, const double* P,
+                const double* Q, int batch, int n, int m, double* X, long long sX_b, long long sX_r,
+                void* ws, size_t ws_bytes)
+{
+    if (!shape_ok(batch, n, m) || !T || !P || !Q || !X) return SSN_ERR_ARGS;
+    if (batch == 0) return SSN_OK;
+    int rc = current_device_ready(nullptr);
+    if (rc) return rc;
+    const size_t need = (size_t)batch * 4 * pad16(m) * pad16(m) * sizeof(double);
+    if (!ws || ws_bytes < need) return SSN_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    rc = hip_status(launch_recovery_fragments(s, P, Q, batch, m, (double*)ws));
+    if (rc) return rc;
+    return hip_status(launch_recover(s, T, sT_b, sT_r, (const double*)ws, batch, n, m, X, sX_b, sX_r));
+}
+
+int ssn_solve(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+              double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
+              long long* iters, int* status, int* counters, void* ws, size_t ws_bytes)
+{
+    if (!shape_ok(batch, n, m) || !T || !X || !omega || !iters || !status || ldo < m) return SSN_ERR_ARGS;
+    if (batch == 0) return SSN_OK;
+    size_t lds = 0;
+    int rc = current_device_ready(&lds);
+    if (rc) return rc;
+    Layout L = layout(batch, n, m);
+    if (!ws || ws_bytes < L.total * sizeof(double)) return SSN_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    double* w = (double*)ws;
+    rc = hip_status(launch_gram_partial(s, T, sT_b, sT_r, batch, n, m, L.chunks, w + L.partial));
+    if (rc) return rc;
+    rc = hip_status(launch_gram_reduce(s, w + L.partial, batch, L.chunks, m, w + L.cr, w + L.ci,
+                                       (long long)m * m, m));
+    if (rc) return rc;
+    rc = hip_status(launch_dual(s, w + L.cr, w + L.ci, (long long)m * m, m, batch, m, omega, sO_b, ldo,
+                                w + L.pf, w + L.qf, iters, status, counters, w + L.dual, lds));
+    if (rc) return rc;
+    rc = hip_status(launch_recovery_fragments(s, w + L.pf, w + L.qf, batch, m, w + L.frag));
+    if (rc) return rc;
+    return hip_status(launch_recover(s, T, sT_b, sT_r, w + L.frag, batch, n, m, X, sX_b, sX_r));
+}
+
+}  // extern "C"

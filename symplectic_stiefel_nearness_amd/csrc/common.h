@@ -1,0 +1,50 @@
+// Shared device helpers for the gfx950 (CDNA4) kernels of the Dual solver.
+//
+// Conventions used by every kernel in this library:
+//  * m = 2k is the column count of a target; small m x m matrices are stored
+//    row-major, padded to mp = 16 * ceil(m / 16) with zero padding, so that the
+//    FP64 MFMA 16x16x4 tiles never need edge masks.
+//  * wave64 everywhere; lane = threadIdx.x & 63.
+//  * FP64 MFMA operand layout (v_mfma_f64_16x16x4f64), lane l = r + 16 q
+//    (r = l & 15, q = l >> 4):
+//        A (16 x 4):  lane holds A[r][q]
+//        B (4 x 16):  lane holds B[q][r]
+//        D (16 x 16): lane holds D[4 q + i][r], i = 0..3
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ssn {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ d4 zero4() {
+    d4 z; z[0] = 0.0; z[1] = 0.0; z[2] = 0.0; z[3] = 0.0;
+    return z;
+}
+
+__host__ __device__ __forceinline__ int pad16(int m) { return (m + 15) & ~15; }
+
+// Deterministic wave-level sum (fixed butterfly order).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// Status codes shared by kernels and the C ABI (include/ssn.h mirrors them).
+enum Status : int {
+    ST_OK = 0,
+    ST_NO_DESCENT = 1,   // Armijo backtracking exhausted
+    ST_NEWTON_CAP = 2,   // Newton iteration cap reached
+    ST_NONFINITE = 3,    // non-finite Gram / residual
+    ST_EIGH = 4,         // Jacobi eigensolver did not converge
+};
+
+}  // namespace ssn

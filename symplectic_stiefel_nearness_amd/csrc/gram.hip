@@ -1,0 +1,225 @@
+// Tall-skinny Gram of a batch of targets T = [A; D] (2n x m, FP64):
+//   Cr = A^T A + D^T D  (symmetric),  Ci = A^T D - D^T A  (skew).
+// Replaces the reference's gram_offdiag / gram_diagonal kernels
+// (src/symplectic_projection/gram_triangle_reflect.cu:1-104) and the chunk loop of
+// gram_chunks (dual_recommended_hvp.py:79-96).
+//
+// Design (gfx950): one workgroup of 8 waves per (row chunk, target). Rows are
+// streamed from HBM through a double-buffered LDS stage of 16 rows of A and D;
+// every wave owns a fixed list of 16x16 output tiles and accumulates them with
+// v_mfma_f64_16x16x4f64 (the row index is the MFMA K dimension). Only the
+// upper tiles of A^T A + D^T D are formed (one accumulator takes both
+// products); all tiles of A^T D are formed and the skew part is taken in the
+// deterministic second stage, which sums the chunk partials in a fixed order.
+#include "common.h"
+
+namespace ssn {
+
+constexpr int GRAM_WAVES = 8;
+constexpr int GRAM_THREADS = GRAM_WAVES * WAVE;
+constexpr int GRAM_ROWS = 16;       // rows of A (and of D) per LDS stage
+
+template <int NT>
+struct GramShape {
+    static constexpr int MP = NT * 16;
+    static constexpr int NCR = NT * (NT + 1) / 2;   // upper tiles of Cr
+    static constexpr int NAD = NT * NT;             // all tiles of A^T D
+    static constexpr int NTOT = NCR + NAD;
+    static constexpr int MAXT = (NTOT + GRAM_WAVES - 1) / GRAM_WAVES;
+    static constexpr int LDS_STRIDE = MP + 2;       // doubles per staged row
+    static constexpr int STAGE = 2 * GRAM_ROWS * LDS_STRIDE;   // A rows then D rows
+    static constexpr int PER_THREAD = (2 * GRAM_ROWS * MP + GRAM_THREADS - 1) / GRAM_THREADS;
+};
+
+__device__ __forceinline__ void tile_of(int t, int nt, int ncr, int& I, int& J, bool& is_cr) {
+    if (t < ncr) {
+        is_cr = true;
+        int col = 0, rem = t;
+        while (rem > col) { rem -= col + 1; ++col; }
+        I = rem; J = col;              // I <= J
+    } else {
+        is_cr = false;
+        const int u = t - ncr;
+        I = u / nt; J = u % nt;
+    }
+}
+
+// partial layout: [batch][chunks][2][MP][MP]; slot 0 = A^T A + D^T D (upper tiles),
+// slot 1 = A^T D (all tiles).
+template <int NT>
+__global__ __launch_bounds__(GRAM_THREADS)
+void gram_partial_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
+                         int n, int m, int chunk_rows, int chunks,
+                         double* __restrict__ partial)
+{
+    using S = GramShape<NT>;
+    constexpr int MP = S::MP;
+    __shared__ double stage[2][S::STAGE];
+
+    const int chunk = blockIdx.x, b = blockIdx.y;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int r = lane & 15, q = lane >> 4;
+    const int row_begin = chunk * chunk_rows;
+    const int row_end = min(n, row_begin + chunk_rows);
+    const double* Tb = T + (long long)b * sT_b;
+
+    d4 acc[S::MAXT];
+#pragma unroll
+    for (int s = 0; s < S::MAXT; ++s) acc[s] = zero4();
+
+    // Static per-thread load slots: element e of a stage -> (half, row, col).
+    double pre[S::PER_THREAD];
+    auto load_stage = [&](int r0) {
+#pragma unroll
+        for (int s = 0; s < S::PER_THREAD; ++s) {
+            const int e = tid + s * GRAM_THREADS;
+            double v = 0.0;
+            if (e < 2 * GRAM_ROWS * MP) {
+                const int half = e / (GRAM_ROWS * MP);
+                const int rem = e - half * GRAM_ROWS * MP;
+                const int rr = rem / MP, cc = rem - (rem / MP) * MP;
+                const int row = r0 + rr;
+                if (row < row_end && cc < m)
+                    v = Tb[(long long)(row + half * n) * sT_r + cc];
+            }
+            pre[s] = v;
+        }
+    };
+    auto store_stage = [&](double* dst) {
+#pragma unroll
+        for (int s = 0; s < S::PER_THREAD; ++s) {
+            const int e = tid + s * GRAM_THREADS;
+            if (e < 2 * GRAM_ROWS * MP) {
+                const int half = e / (GRAM_ROWS * MP);
+                const int rem = e - half * GRAM_ROWS * MP;
+                const int rr = rem / MP, cc = rem - (rem / MP) * MP;
+                dst[(half * GRAM_ROWS + rr) * S::LDS_STRIDE + cc] = pre[s];
+            }
+        }
+    };
+
+    // tile coordinates of this wave, fixed for the whole chunk
+    int tI[S::MAXT], tJ[S::MAXT];
+    bool tcr[S::MAXT];
+#pragma unroll
+    for (int s = 0; s < S::MAXT; ++s) {
+        const int t = wave + s * GRAM_WAVES;
+        tile_of(t < S::NTOT ? t : 0, NT, S::NCR, tI[s], tJ[s], tcr[s]);
+    }
+
+    // one barrier per stage: the buffer written at stage st+1 is the other one, and
+    // the barrier of stage st+1 orders every read of stage st before the write at st+2
+    const int nstages = (row_end - row_begin + GRAM_ROWS - 1) / GRAM_ROWS;
+    if (nstages > 0) load_stage(row_begin);
+    for (int st = 0; st < nstages; ++st) {
+        double* cur = stage[st & 1];
+        store_stage(cur);
+        __syncthreads();
+        if (st + 1 < nstages) load_stage(row_begin + (st + 1) * GRAM_ROWS);
+        const double* sA = cur;
+        const double* sD = cur + GRAM_ROWS * S::LDS_STRIDE;
+#pragma unroll
+        for (int kq = 0; kq < GRAM_ROWS / 4; ++kq) {
+            const int row = (kq * 4 + q) * S::LDS_STRIDE + r;
+#pragma unroll
+            for (int s = 0; s < S::MAXT; ++s) {
+                if (wave + s * GRAM_WAVES < S::NTOT) {
+                    const int I = tI[s] * 16, J = tJ[s] * 16;
+                    const double aI = sA[row + I];
+                    const double dJ = sD[row + J];
+                    if (tcr[s]) {
+                        const double aJ = sA[row + J];
+                        const double dI = sD[row + I];
+                        acc[s] = mfma(aI, aJ, acc[s]);
+                        acc[s] = mfma(dI, dJ, acc[s]);
+                    } else {
+                        acc[s] = mfma(aI, dJ, acc[s]);
+                    }
+                }
+            }
+        }
+    }
+
+    double* out = partial + ((long long)b * chunks + chunk) * 2 * MP * MP;
+#pragma unroll
+    for (int s = 0; s < S::MAXT; ++s) {
+        if (wave + s * GRAM_WAVES < S::NTOT) {
+            double* o = out + (tcr[s] ? 0 : MP * MP);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                o[(tI[s] * 16 + 4 * q + i) * MP + tJ[s] * 16 + r] = acc[s][i];
+        }
+    }
+}
+
+// Deterministic second stage: Cr, Ci [batch][m][m] (leading dimension ldc) from the
+// chunk partials, summed in ascending chunk order. Cr is mirrored from its upper
+// triangle, Ci = S - S^T from the A^T D sums, so symmetry / skewness is exact.
+__global__ void gram_reduce_kernel(const double* __restrict__ partial, int chunks, int m, int mp,
+                                   double* __restrict__ Cr, double* __restrict__ Ci,
+                                   long long sC_b, int ldc)
+{
+    const int b = blockIdx.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * m) return;
+    const int i = e / m, j = e - (e / m) * m;
+    const int lo = min(i, j), hi = max(i, j);
+    const double* p = partial + (long long)b * chunks * 2 * mp * mp;
+    double cr = 0.0, ad = 0.0, da = 0.0;
+    for (int c = 0; c < chunks; ++c) {
+        const double* pc = p + (long long)c * 2 * mp * mp;
+        cr += pc[lo * mp + hi];
+        ad += pc[mp * mp + lo * mp + hi];
+        da += pc[mp * mp + hi * mp + lo];
+    }
+    const double ci_upper = ad - da;   // Ci[lo][hi]
+    Cr[(long long)b * sC_b + (long long)i * ldc + j] = cr;
+    Ci[(long long)b * sC_b + (long long)i * ldc + j] = (i == j) ? 0.0 : (i < j ? ci_upper : -ci_upper);
+}
+
+}  // namespace ssn
+
+// ------------------------------------------------------------------ launchers
+namespace ssn {
+
+int gram_chunk_count(int batch, int n) {
+    // enough workgroups to cover the chip about twice; at least 256 rows per chunk
+    const int target_wgs = 512;
+    int chunks = (target_wgs + batch - 1) / batch;
+    const int max_chunks = (n + 255) / 256;
+    if (chunks > max_chunks) chunks = max_chunks;
+    if (chunks < 1) chunks = 1;
+    return chunks;
+}
+
+hipError_t launch_gram_partial(hipStream_t stream, const double* T, long long sT_b, long long sT_r,
+                               int batch, int n, int m, int chunks, double* partial)
+{
+    const int mp = pad16(m);
+    const int nt = mp / 16;
+    int chunk_rows = (n + chunks - 1) / chunks;
+    chunk_rows = (chunk_rows + GRAM_ROWS - 1) / GRAM_ROWS * GRAM_ROWS;
+    dim3 grid(chunks, batch), block(GRAM_THREADS);
+#define SSN_GRAM_CASE(NTV) \
+    case NTV: hipLaunchKernelGGL(gram_partial_kernel<NTV>, grid, block, 0, stream, \
+                                 T, sT_b, sT_r, n, m, chunk_rows, chunks, partial); break;
+    switch (nt) {
+        SSN_GRAM_CASE(1) SSN_GRAM_CASE(2) SSN_GRAM_CASE(3) SSN_GRAM_CASE(4)
+        SSN_GRAM_CASE(5) SSN_GRAM_CASE(6) SSN_GRAM_CASE(7) SSN_GRAM_CASE(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef SSN_GRAM_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_gram_reduce(hipStream_t stream, const double* partial, int batch, int chunks, int m,
+                              double* Cr, double* Ci, long long sC_b, int ldc)
+{
+    const int mp = pad16(m);
+    dim3 grid((m * m + 255) / 256, batch), block(256);
+    hipLaunchKernelGGL(gram_reduce_kernel, grid, block, 0, stream, partial, chunks, m, mp,
+                       Cr, Ci, sC_b, ldc);
+    return hipGetLastError();
+}
+
+}  // namespace ssn

@@ -1,0 +1,121 @@
+// Recovery of the primal solution X = [A P^T + D Q^T ; D P^T - A Q^T] for a batch
+// of targets T = [A; D], i.e. X_c = B N^H with B = A + iD, N = P + iQ.
+// Replaces recover_fused (src/symplectic_projection/recovery_fused.cu:1-41) and the
+// chunk loop of recover_matmul (dual_recommended_hvp.py:194-228).
+//
+// Row r of [X_top | X_bot] is the row [a_r | d_r] times the 2mp x 2mp matrix
+//     M = [[P^T, -Q^T], [Q^T, P^T]],
+// so the kernel is a streaming GEMM (n x 2mp) * (2mp x 2mp) on FP64 MFMA. M is
+// supplied pre-swizzled in MFMA B-fragment order (built once per target by
+// `build_recovery_fragments`), so every B-operand load of a wave is one
+// contiguous 512-byte read that all workgroups of the target share through L2.
+#include "common.h"
+
+namespace ssn {
+
+constexpr int REC_WAVES = 4;
+constexpr int REC_THREADS = REC_WAVES * WAVE;
+constexpr int REC_ROWS = REC_WAVES * 16;    // rows of A (and of D) per workgroup
+
+// frag[b][ks][J][lane] = M[4 ks + (lane >> 4)][16 J + (lane & 15)], ks < 2mp/4, J < 2mp/16.
+// P, Q are [batch][mp][mp] row-major with zero padding.
+__global__ void build_recovery_fragments(const double* __restrict__ P, const double* __restrict__ Q,
+                                         int mp, double* __restrict__ frag)
+{
+    const int b = blockIdx.y;
+    const int m2 = 2 * mp;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m2 * m2) return;
+    const int lane = e & 63;
+    const int J = (e >> 6) % (m2 / 16);
+    const int ks = (e >> 6) / (m2 / 16);
+    const int k = 4 * ks + (lane >> 4), j = 16 * J + (lane & 15);
+    const double* Pb = P + (long long)b * mp * mp;
+    const double* Qb = Q + (long long)b * mp * mp;
+    double v;
+    if (k < mp) v = (j < mp) ? Pb[j * mp + k] : -Qb[(j - mp) * mp + k];           // P^T | -Q^T
+    else        v = (j < mp) ? Qb[j * mp + (k - mp)] : Pb[(j - mp) * mp + (k - mp)]; // Q^T |  P^T
+    frag[(long long)b * m2 * m2 + e] = v;
+}
+
+template <int NT>
+__global__ __launch_bounds__(REC_THREADS)
+void recover_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
+                    const double* __restrict__ frag, int n, int m,
+                    double* __restrict__ X, long long sX_b, long long sX_r)
+{
+    constexpr int MP = NT * 16;
+    constexpr int NJ = 2 * NT;          // output tiles per row block (top + bottom)
+    constexpr int KS = 2 * MP / 4;      // k-steps
+    const int b = blockIdx.y;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = lane & 15, q = lane >> 4;
+    const int row = blockIdx.x * REC_ROWS + wave * 16 + r;      // A-operand row of this lane
+    const bool row_ok = row < n;
+    const double* Tb = T + (long long)b * sT_b;
+    const double* Arow = Tb + (long long)row * sT_r;
+    const double* Drow = Tb + (long long)(row + n) * sT_r;
+    const double* F = frag + (long long)b * (2 * MP) * (2 * MP);
+
+    d4 acc[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[j] = zero4();
+
+#pragma unroll 2
+    for (int ks = 0; ks < KS; ++ks) {
+        const int k = 4 * ks + q;
+        double a = 0.0;
+        if (row_ok) {
+            if (k < MP) { if (k < m) a = Arow[k]; }
+            else if (k - MP < m) a = Drow[k - MP];
+        }
+        const double* Fk = F + (long long)ks * NJ * 64 + lane;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc[j] = mfma(a, Fk[j * 64], acc[j]);
+    }
+
+    // lane holds D[4q + i][r] of each tile: rows wave*16 + 4q + i, column 16 J + r
+    double* Xb = X + (long long)b * sX_b;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        const bool top = j < NT;
+        const int col = (top ? j : j - NT) * 16 + r;
+        if (col >= m) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int orow = blockIdx.x * REC_ROWS + wave * 16 + 4 * q + i;
+            if (orow < n)
+                Xb[(long long)(top ? orow : orow + n) * sX_r + col] = acc[j][i];
+        }
+    }
+}
+
+hipError_t launch_recovery_fragments(hipStream_t stream, const double* P, const double* Q,
+                                     int batch, int m, double* frag)
+{
+    const int mp = pad16(m);
+    const int total = 4 * mp * mp;
+    dim3 grid((total + 255) / 256, batch), block(256);
+    hipLaunchKernelGGL(build_recovery_fragments, grid, block, 0, stream, P, Q, mp, frag);
+    return hipGetLastError();
+}
+
+hipError_t launch_recover(hipStream_t stream, const double* T, long long sT_b, long long sT_r,
+                          const double* frag, int batch, int n, int m,
+                          double* X, long long sX_b, long long sX_r)
+{
+    const int nt = pad16(m) / 16;
+    dim3 grid((n + REC_ROWS - 1) / REC_ROWS, batch), block(REC_THREADS);
+#define SSN_REC_CASE(NTV) \
+    case NTV: hipLaunchKernelGGL(recover_kernel<NTV>, grid, block, 0, stream, \
+                                 T, sT_b, sT_r, frag, n, m, X, sX_b, sX_r); break;
+    switch (nt) {
+        SSN_REC_CASE(1) SSN_REC_CASE(2) SSN_REC_CASE(3) SSN_REC_CASE(4)
+        SSN_REC_CASE(5) SSN_REC_CASE(6) SSN_REC_CASE(7) SSN_REC_CASE(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef SSN_REC_CASE
+    return hipGetLastError();
+}
+
+}  // namespace ssn

@@ -1,0 +1,210 @@
+"""Host driver of the Dual solver on MI355X: the drop-in for
+`symplectic_projection.dual_recommended_hvp.solve` (src/symplectic_projection/dual_recommended_hvp.py:231-242).
+
+`solve(T)` keeps the reference call surface: float64 targets [batch, 2n, 2k] in,
+`(X, omega, iterations)` out. CPU input gives CPU outputs (the reference protocol,
+PCIe included); device input gives device outputs with nothing leaving HBM.
+All arithmetic runs in libssn.so (HIP, gfx950); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+ROW_CHUNK = 1024                       # dual_recommended_hvp.py:13 (kept for reference; rows stream whole)
+FEAS_TOL = 1e-8                        # dual_recommended_hvp.py:15
+MAX_NEWTON, MAX_CG, MAX_BACKTRACK = 100, 150, 40   # dual_recommended_hvp.py:16
+PINNED_OUTPUT_MIN_BYTES = 64 * 2 ** 20  # dual_recommended_hvp.py:18
+MAX_ORDER = 128                        # largest 2k handled by this build
+
+MSG_NO_DESCENT = "批量线搜索未找到下降步"      # dual_recommended_hvp.py:190 (line search found no descent step)
+MSG_NEWTON_CAP = "达到批量 Newton 迭代上限"     # dual_recommended_hvp.py:191 (Newton iteration cap reached)
+
+_state = {"initialized": False}
+
+
+def initialize_kernels():
+    """Load libssn.so and query the device once per process (__init__.py:10-19)."""
+    if _state["initialized"]:
+        return
+    lib = _lib.load()
+    if torch.cuda.is_available():
+        _lib.check(lib.ssn_init(), "ssn_init")
+    _state["initialized"] = True
+
+
+def _require_ready():
+    if not _state["initialized"]:
+        raise RuntimeError("initialize_kernels() must be called once in this process before solve()")
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device is visible: the Dual solver runs only on the GPU")
+    return _lib.load()
+
+
+def _check_targets(T):
+    assert isinstance(T, torch.Tensor) and T.dtype == torch.float64, "T must be a float64 tensor"
+    assert T.ndim == 3, "T must have shape [batch, 2n, 2k]"
+    batch, rows, m = T.shape
+    assert rows % 2 == 0 and rows >= 2, "T must have an even number of rows (2n)"
+    assert m % 2 == 0 and m >= 2, "T must have an even number of columns (2k)"
+    if m > MAX_ORDER:
+        raise ValueError(f"2k = {m} exceeds the largest order supported by this build ({MAX_ORDER})")
+    return batch, rows // 2, m
+
+
+def _stream():
+    return ctypes_stream(torch.cuda.current_stream())
+
+
+def ctypes_stream(stream):
+    return stream.cuda_stream
+
+
+def _raise_for_status(status):
+    """Batch-wide exceptions of the reference (dual_recommended_hvp.py:189-191)."""
+    st = status.cpu()
+    if bool((st == _lib.ST_NO_DESCENT).any()):
+        raise RuntimeError(MSG_NO_DESCENT)
+    if bool((st == _lib.ST_NEWTON_CAP).any()):
+        raise RuntimeError(MSG_NEWTON_CAP)
+    if bool((st == _lib.ST_NONFINITE).any()):
+        raise RuntimeError("non-finite dual residual (non-finite target?)")
+    if bool((st == _lib.ST_EIGH).any()):
+        raise RuntimeError("Jacobi eigensolver did not converge")
+
+
+def _workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 8), dtype=torch.uint8, device=device)
+
+
+@torch.no_grad()
+def solve_device(T, return_stats=False, check=True):
+    """Device-resident solve: T is a float64 HIP tensor [batch, 2n, 2k]; outputs stay on the device.
+
+    Returns (X, omega, iterations[, stats]) where stats = dict(status, counters[batch, 4]
+    with columns newton, cg, backtracks, hvps).
+    """
+    lib = _require_ready()
+    batch, n, m = _check_targets(T)
+    assert T.is_cuda, "solve_device expects a device tensor"
+    dev = T.device
+    if T.stride(-1) != 1:
+        T = T.contiguous()
+    X = torch.empty((batch, 2 * n, m), dtype=torch.float64, device=dev)
+    omega = torch.empty((batch, m, m), dtype=torch.float64, device=dev)
+    iters = torch.zeros(batch, dtype=torch.int64, device=dev)
+    status = torch.zeros(batch, dtype=torch.int32, device=dev)
+    counters = torch.zeros((batch, 4), dtype=torch.int32, device=dev)
+    if batch:
+        nbytes = lib.ssn_workspace_bytes(batch, n, m)
+        ws = _workspace(nbytes, dev)
+        with torch.cuda.device(dev):
+            rc = lib.ssn_solve(_stream(), T.data_ptr(), T.stride(0), T.stride(1), batch, n, m,
+                               X.data_ptr(), X.stride(0), X.stride(1),
+                               omega.data_ptr(), omega.stride(0), omega.stride(1),
+                               iters.data_ptr(), status.data_ptr(), counters.data_ptr(),
+                               ws.data_ptr(), ws.numel())
+        _lib.check(rc, "ssn_solve")
+    if check:
+        _raise_for_status(status)
+    if return_stats:
+        return X, omega, iters, dict(status=status, counters=counters)
+    return X, omega, iters
+
+
+@torch.no_grad()
+def solve(T, return_stats=False):
+    """(X, omega, iterations) for float64 targets T [batch, 2n, 2k] (dual_recommended_hvp.py:231-242).
+
+    CPU input -> CPU outputs (H2D and D2H included, X pinned when >= 64 MiB as in
+    dual_recommended_hvp.py:204-205); device input -> device outputs.
+    """
+    _require_ready()
+    _check_targets(T)
+    if T.is_cuda:
+        return solve_device(T, return_stats=return_stats)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    Td = T.to(dev)
+    out = solve_device(Td, return_stats=return_stats)
+    X, omega, iters = out[:3]
+    direct = X.numel() * X.element_size() >= PINNED_OUTPUT_MIN_BYTES
+    Xh = torch.empty(X.shape, dtype=X.dtype, pin_memory=direct)
+    Xh.copy_(X)
+    res = (Xh, omega.cpu(), iters.cpu())
+    if return_stats:
+        stats = {k: v.cpu() for k, v in out[3].items()}
+        return res + (stats,)
+    return res
+
+
+# ---------------------------------------------------------------- the three stages, separately
+
+@torch.no_grad()
+def gram_chunks(T):
+    """Device Cr = A^T A + D^T D, Ci = A^T D - D^T A for device T (dual_recommended_hvp.py:79-96)."""
+    lib = _require_ready()
+    batch, n, m = _check_targets(T)
+    if T.stride(-1) != 1:
+        T = T.contiguous()
+    Cr = torch.empty((batch, m, m), dtype=torch.float64, device=T.device)
+    Ci = torch.empty_like(Cr)
+    if batch:
+        ws = _workspace(lib.ssn_workspace_bytes(batch, n, m), T.device)
+        rc = lib.ssn_gram(_stream(), T.data_ptr(), T.stride(0), T.stride(1), batch, n, m,
+                          Cr.data_ptr(), Ci.data_ptr(), Cr.stride(0), Cr.stride(1),
+                          ws.data_ptr(), ws.numel())
+        _lib.check(rc, "ssn_gram")
+    return Cr, Ci
+
+
+@torch.no_grad()
+def dual_batch(Cr, Ci, check=True):
+    """Dual Newton-CG on device Gram parts (dual_recommended_hvp.py:99-191).
+
+    Returns (omega, P, Q, iterations, stats); P, Q are [batch, mp, mp] (mp = 16 ceil(m/16),
+    zero padded) in the original basis, the reference's `final` cache (:131-133).
+    """
+    lib = _require_ready()
+    assert Cr.shape == Ci.shape and Cr.ndim == 3 and Cr.shape[-1] == Cr.shape[-2]
+    assert Cr.dtype == Ci.dtype == torch.float64 and Cr.is_cuda and Ci.is_cuda
+    batch, m, _ = Cr.shape
+    assert m % 2 == 0 and 2 <= m <= MAX_ORDER
+    Cr = Cr.contiguous()
+    Ci = Ci.contiguous()
+    mp = lib.ssn_padded_order(m)
+    dev = Cr.device
+    omega = torch.empty((batch, m, m), dtype=torch.float64, device=dev)
+    P = torch.empty((batch, mp, mp), dtype=torch.float64, device=dev)
+    Q = torch.empty_like(P)
+    iters = torch.zeros(batch, dtype=torch.int64, device=dev)
+    status = torch.zeros(batch, dtype=torch.int32, device=dev)
+    counters = torch.zeros((batch, 4), dtype=torch.int32, device=dev)
+    if batch:
+        ws = _workspace(lib.ssn_workspace_bytes(batch, 1, m), dev)
+        rc = lib.ssn_dual(_stream(), Cr.data_ptr(), Ci.data_ptr(), Cr.stride(0), Cr.stride(1), batch, m,
+                          omega.data_ptr(), omega.stride(0), omega.stride(1), P.data_ptr(), Q.data_ptr(),
+                          iters.data_ptr(), status.data_ptr(), counters.data_ptr(),
+                          ws.data_ptr(), ws.numel())
+        _lib.check(rc, "ssn_dual")
+    if check:
+        _raise_for_status(status)
+    return omega, P, Q, iters, dict(status=status, counters=counters)
+
+
+@torch.no_grad()
+def recover_matmul(T, P, Q):
+    """X = [A P^T + D Q^T; D P^T - A Q^T] on device (dual_recommended_hvp.py:194-228)."""
+    lib = _require_ready()
+    batch, n, m = _check_targets(T)
+    mp = lib.ssn_padded_order(m)
+    assert P.shape == Q.shape == (batch, mp, mp) and P.is_contiguous() and Q.is_contiguous()
+    if T.stride(-1) != 1:
+        T = T.contiguous()
+    X = torch.empty((batch, 2 * n, m), dtype=torch.float64, device=T.device)
+    if batch:
+        ws = _workspace(batch * 4 * mp * mp * 8, T.device)
+        rc = lib.ssn_recover(_stream(), T.data_ptr(), T.stride(0), T.stride(1), P.data_ptr(), Q.data_ptr(),
+                             batch, n, m, X.data_ptr(), X.stride(0), X.stride(1), ws.data_ptr(), ws.numel())
+        _lib.check(rc, "ssn_recover")
+    return X

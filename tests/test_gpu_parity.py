@@ -1,0 +1,175 @@
+"""GPU parity of the HIP Dual solver against the CPU FP64 oracle (restatement of the reference).
+
+Tolerances: X and omega within 1e-10 relative (Frobenius, per target)
+of the oracle on the same seeded targets; acceptance r_F <= 1e-8, r_G <= 1e-9, gap <= 1e-8
+recomputed on the CPU; Gram and recovery within 1e-13 relative of NumPy FP64.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import acceptance, dual as odual
+from symplectic_stiefel_nearness_amd import inputs
+
+pytestmark = pytest.mark.gpu
+
+PARITY_TOL = 1e-10
+
+
+def rel(a, b):
+    a = np.asarray(a); b = np.asarray(b)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def say(*args):
+    print(*args, flush=True)
+    sys.stdout.flush()
+
+
+def test_device_properties(ssn):
+    props = torch.cuda.get_device_properties(0)
+    say("device", props.name, "CUs", props.multi_processor_count, "gcnArch", getattr(props, "gcnArchName", "?"))
+    lib = ssn.dual._lib.load()
+    say("ssn_version", lib.ssn_version().decode())
+    assert lib.ssn_init() == 0
+
+
+@pytest.mark.parametrize("n,m,batch", [(7, 2, 1), (64, 16, 2), (300, 20, 3), (1000, 96, 2), (513, 128, 2)])
+def test_gram_parity(ssn, n, m, batch):
+    say("gram", n, m, batch)
+    g = torch.Generator().manual_seed(n * 1000 + m)
+    T = torch.randn(batch, 2 * n, m, dtype=torch.float64, generator=g) / math.sqrt(2 * n)
+    Cr, Ci = ssn.gram_chunks(T.cuda())
+    torch.cuda.synchronize()
+    Tn = T.numpy()
+    A, D = Tn[:, :n], Tn[:, n:]
+    Cr_ref = np.swapaxes(A, 1, 2) @ A + np.swapaxes(D, 1, 2) @ D
+    Ci_ref = np.swapaxes(A, 1, 2) @ D - np.swapaxes(D, 1, 2) @ A
+    Crh, Cih = Cr.cpu().numpy(), Ci.cpu().numpy()
+    assert rel(Crh, Cr_ref) < 1e-13
+    assert rel(Cih, Ci_ref) < 1e-13
+    assert np.array_equal(Crh, np.swapaxes(Crh, 1, 2))          # exact symmetry
+    assert np.array_equal(Cih, -np.swapaxes(Cih, 1, 2))         # exact skewness
+
+
+@pytest.mark.parametrize("n,m,batch", [(5, 2, 1), (64, 16, 2), (200, 20, 2), (1000, 96, 2), (300, 128, 1)])
+def test_recover_parity(ssn, n, m, batch):
+    say("recover", n, m, batch)
+    lib = ssn.dual._lib.load()
+    mp = lib.ssn_padded_order(m)
+    g = torch.Generator().manual_seed(7 + n + m)
+    T = torch.randn(batch, 2 * n, m, dtype=torch.float64, generator=g)
+    P = torch.zeros(batch, mp, mp, dtype=torch.float64)
+    Q = torch.zeros_like(P)
+    P[:, :m, :m] = torch.randn(batch, m, m, dtype=torch.float64, generator=g)
+    Q[:, :m, :m] = torch.randn(batch, m, m, dtype=torch.float64, generator=g)
+    X = ssn.recover_matmul(T.cuda(), P.cuda(), Q.cuda()).cpu().numpy()
+    Tn, Pn, Qn = T.numpy(), P[:, :m, :m].numpy(), Q[:, :m, :m].numpy()
+    X_ref = odual.recover(Tn, Pn, Qn)
+    assert rel(X, X_ref) < 1e-13
+
+
+def _solve_and_compare(ssn, T, expect_iters=True):
+    X, omega, iters, stats = ssn.solve(T, return_stats=True)
+    Xo, omo, ito, sto = odual.solve(T.numpy(), return_stats=True)
+    cnt = stats["counters"].numpy()
+    for j in range(T.shape[0]):
+        ex, eo = rel(X[j].numpy(), Xo[j]), rel(omega[j].numpy(), omo[j])
+        say(f"  target {j}: newton {int(iters[j])}/{int(ito[j])} cg {cnt[j,1]}/{sto.cg[j]} "
+            f"bt {cnt[j,2]}/{sto.backtracks[j]}  relX {ex:.2e} relOmega {eo:.2e}")
+        assert ex <= PARITY_TOL and eo <= PARITY_TOL
+        if expect_iters:
+            assert int(iters[j]) == int(ito[j])
+            assert cnt[j, 1] == sto.cg[j] and cnt[j, 2] == sto.backtracks[j]
+    recs = acceptance.inspect_output(T.numpy(), X.numpy())
+    for r in recs:
+        assert acceptance.numeric_pass(r), r
+    return X, omega, iters
+
+
+def test_solve_config1_plain(ssn):
+    """n=256, k=8, one target (case n256_k8)."""
+    say("solve n256_k8 x1")
+    T = inputs.make_case("n256_k8", 1)
+    _solve_and_compare(ssn, T)
+
+
+def test_solve_config2_subset(ssn):
+    """n=2048, k=16 (reference case n2048_k16), first 8 of the 64 seeded targets."""
+    say("solve n2048_k16 x8")
+    T = inputs.make_case("n2048_k16", 8)
+    _solve_and_compare(ssn, T)
+
+
+def test_solve_metric_subset(ssn):
+    """n=4096, k=48 (reference case n4096_k48), first 4 seeded targets."""
+    say("solve n4096_k48 x4")
+    T = inputs.make_case("n4096_k48", 4)
+    _solve_and_compare(ssn, T)
+
+
+def test_solve_order128(ssn):
+    """n=8192, k=64: m = 128, the largest order (two seeded targets)."""
+    say("solve n8192_k64 x2")
+    T = inputs.make_case("n8192_k64", 2)
+    _solve_and_compare(ssn, T)
+
+
+@pytest.mark.parametrize("n,k", [(3, 1), (40, 3), (150, 10), (500, 25)])
+def test_solve_unpadded_orders(ssn, n, k):
+    """Orders m = 2k that are not multiples of 16 exercise the zero padding."""
+    say("solve padded", n, k)
+    T = inputs.make_targets(n, k, 2, seeds=[11 + n, 12 + n])
+    _solve_and_compare(ssn, T)
+
+
+def test_known_answer_orthosymplectic(ssn):
+    """T = c S with S orthosymplectic: X = S, omega = (c - 1) J; c = 1 needs 0 Newton updates."""
+    n, k = 64, 8
+    g = np.random.default_rng(5)
+    Z = g.standard_normal((n, k)) + 1j * g.standard_normal((n, k))
+    Qc, _ = np.linalg.qr(Z)
+    S = np.block([[Qc.real, -Qc.imag], [Qc.imag, Qc.real]])
+    J = acceptance.standard_j(k)
+    for c in (1.0, 0.5, 1.7):
+        T = torch.from_numpy((c * S)[None].copy())
+        X, omega, iters = ssn.solve(T)
+        assert rel(X[0].numpy(), S) < 1e-10
+        assert np.abs(omega[0].numpy() - (c - 1) * J).max() < 1e-10
+        if c == 1.0:
+            assert int(iters[0]) == 0
+
+
+def test_empty_batch(ssn):
+    X, omega, iters = ssn.solve(torch.empty(0, 8, 4, dtype=torch.float64))
+    assert X.shape == (0, 8, 4) and omega.shape == (0, 4, 4) and iters.shape == (0,)
+
+
+def test_device_resident_matches_host(ssn):
+    T = inputs.make_case("n256_k8", 3)
+    Xh, omh, ith = ssn.solve(T)
+    Xd, omd, itd = ssn.solve(T.cuda())
+    assert Xd.is_cuda and omd.is_cuda
+    assert torch.equal(Xd.cpu(), Xh) and torch.equal(omd.cpu(), omh) and torch.equal(itd.cpu(), ith)
+
+
+def test_run_to_run_bitwise(ssn):
+    T = inputs.make_case("n2048_k16", 2).cuda()
+    a = ssn.solve(T)
+    b = ssn.solve(T)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)
+
+
+def test_bad_inputs(ssn):
+    with pytest.raises(AssertionError):
+        ssn.solve(torch.zeros(1, 8, 4, dtype=torch.float32))
+    with pytest.raises(AssertionError):
+        ssn.solve(torch.zeros(8, 4, dtype=torch.float64))
+    with pytest.raises(AssertionError):
+        ssn.solve(torch.zeros(1, 7, 4, dtype=torch.float64))
+    with pytest.raises(ValueError):
+        ssn.solve(torch.zeros(1, 400, 130, dtype=torch.float64))
