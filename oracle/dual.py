@@ -103,7 +103,7 @@ def outer_skew(P, Q, U, V):
 
 def sandwich_front(values, Ci, P, Q):
     """Yr = P diag(values) - Q Ci, Yi = P Ci + Q diag(values) (hvp_fused.cu:39-80)."""
-    d = values[:, None, :]
+    d = values[..., None, :]
     return P * d - Q @ Ci, P @ Ci + Q * d
 
 

@@ -9,7 +9,7 @@
 //    (r = l & 15, q = l >> 4):
 //        A (16 x 4):  lane holds A[r][q]
 //        B (4 x 16):  lane holds B[q][r]
-//        D (16 x 16): lane holds D[4 q + i][r], i = 0..3
+//        D (16 x 16): lane holds D[q + 4 i][r], i = 0..3 (rows strided by 4)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

@@ -108,7 +108,7 @@ __device__ void gemm(const Ctx& c, const double* A, const double* B, F f) {
         d4 acc = zero4();
         tile_mac<TA, TB>(A, B, c.mp, I, J, c.r, c.q, acc);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) f(I * 16 + 4 * c.q + i, J * 16 + c.r, acc[i]);
+        for (int i = 0; i < 4; ++i) f(I * 16 + c.q + 4 * i, J * 16 + c.r, acc[i]);
     }
 }
 
@@ -123,7 +123,7 @@ __device__ void gemm_diff(const Ctx& c, const double* A1, const double* B1,
         tile_mac<false, false>(A1, B1, c.mp, I, J, c.r, c.q, acc);
         tile_mac_neg(A2, B2, c.mp, I, J, c.r, c.q, acc);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) f(I * 16 + 4 * c.q + i, J * 16 + c.r, acc[i]);
+        for (int i = 0; i < 4; ++i) f(I * 16 + c.q + 4 * i, J * 16 + c.r, acc[i]);
     }
 }
 
@@ -149,7 +149,7 @@ __device__ void gemm_nt2(const Ctx& c, const double* A1, const double* B1,
             for (int u = 0; u < 4; ++u) acc = mfma(a[u], b[u], acc);
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) f(I * 16 + 4 * c.q + i, J * 16 + c.r, acc[i]);
+        for (int i = 0; i < 4; ++i) f(I * 16 + c.q + 4 * i, J * 16 + c.r, acc[i]);
     }
 }
 

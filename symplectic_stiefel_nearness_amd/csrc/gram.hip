@@ -147,7 +147,7 @@ void gram_partial_kernel(const double* __restrict__ T, long long sT_b, long long
             double* o = out + (tcr[s] ? 0 : MP * MP);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                o[(tI[s] * 16 + 4 * q + i) * MP + tJ[s] * 16 + r] = acc[s][i];
+                o[(tI[s] * 16 + q + 4 * i) * MP + tJ[s] * 16 + r] = acc[s][i];
         }
     }
 }

@@ -74,7 +74,7 @@ void recover_kernel(const double* __restrict__ T, long long sT_b, long long sT_r
         for (int j = 0; j < NJ; ++j) acc[j] = mfma(a, Fk[j * 64], acc[j]);
     }
 
-    // lane holds D[4q + i][r] of each tile: rows wave*16 + 4q + i, column 16 J + r
+    // lane holds D[q + 4i][r] of each tile: rows wave*16 + q + 4i, column 16 J + r
     double* Xb = X + (long long)b * sX_b;
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
@@ -83,7 +83,7 @@ void recover_kernel(const double* __restrict__ T, long long sT_b, long long sT_r
         if (col >= m) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int orow = blockIdx.x * REC_ROWS + wave * 16 + 4 * q + i;
+            const int orow = blockIdx.x * REC_ROWS + wave * 16 + q + 4 * i;
             if (orow < n)
                 Xb[(long long)(top ? orow : orow + n) * sX_r + col] = acc[j][i];
         }
