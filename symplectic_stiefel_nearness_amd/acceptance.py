@@ -7,6 +7,8 @@ projection of `baselines/geometry.py:22-39` and the thresholds of `reproduce.py:
     r_G   = ||Proj_Y(2 (Y - T))|| / (2 max(1, ||T||))
     delta = |objective - dual| / max(1, |objective|),  dual = ||T||^2 - ||L^-1 B^H||^2 - <omega, J>
 
+(||L^-1 B^H||^2 is evaluated as Re tr((I + i omega)^-1 B^H B), the same number.)
+
 with omega the multiplier of the projection / 2 and L the Cholesky factor of I + i omega.
 This is a checker for reports and benchmarks (torch.linalg on FP64); it is not on the
 solver's hot path.
@@ -65,8 +67,13 @@ def inspect(T, Y):
     kc = eye.to(torch.complex128) + 1j * omega.to(torch.complex128)
     b = torch.complex(T[:, :n], T[:, n:])
     factor, info = torch.linalg.cholesky_ex(kc)
-    solved = torch.linalg.solve_triangular(factor, b.mH, upper=False)
-    dual = (T * T).sum((-2, -1)) - (solved.abs() ** 2).sum((-2, -1)) - (omega * jm).sum((-2, -1))
+    # ||L^-1 B^H||_F^2 = Re tr((L L^H)^-1 B^H B): the same quantity through the m x m
+    # complex Gram, which keeps the device work at m x m (a batched n-column triangular
+    # solve needs more library workspace than hipBLAS provides at n = 4096)
+    gram = b.mH.contiguous() @ b
+    kinv = torch.cholesky_inverse(factor)
+    solved_sq = (kinv * gram.mT).sum((-2, -1)).real
+    dual = (T * T).sum((-2, -1)) - solved_sq - (omega * jm).sum((-2, -1))
     delta = ((objective - dual) / objective.abs().clamp_min(1.0)).abs()
     delta = torch.where(info == 0, delta, torch.full_like(delta, float("nan")))
     ok = torch.isfinite(Y).flatten(1).all(1) & (values.min(-1).values > 0)

@@ -31,11 +31,33 @@ __device__ __forceinline__ d4 zero4() {
 
 __host__ __device__ __forceinline__ int pad16(int m) { return (m + 15) & ~15; }
 
-// Deterministic wave-level sum (fixed butterfly order).
+// DPP move of a double (two 32-bit halves); lanes without a source read 0.
+template <int CTRL, int ROWMASK, int BANKMASK>
+__device__ __forceinline__ double dpp(double v) {
+    const long long x = __double_as_longlong(v);
+    int lo = (int)(x & 0xffffffffLL), hi = (int)(x >> 32);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWMASK, BANKMASK, true);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWMASK, BANKMASK, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Deterministic wave-level sum with DPP (fixed order): quad butterflies, 16-lane row
+// rotations, then row broadcasts; lane 63 ends with the total, returned wave-uniform.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
-    return v;
+    v += dpp<0xB1, 0xF, 0xF>(v);    // quad_perm [1,0,3,2]
+    v += dpp<0x4E, 0xF, 0xF>(v);    // quad_perm [2,3,0,1]
+    v += dpp<0x124, 0xF, 0xF>(v);   // row_ror:4
+    v += dpp<0x128, 0xF, 0xF>(v);   // row_ror:8
+    v += dpp<0x142, 0xA, 0xF>(v);   // row_bcast:15 into rows 1, 3
+    v += dpp<0x143, 0xC, 0xF>(v);   // row_bcast:31 into rows 2, 3
+    return readlane_d(v, 63);
 }
 
 // Status codes shared by kernels and the C ABI (include/ssn.h mirrors them).

@@ -8,11 +8,22 @@
 // the single-target trajectory (a masked target only ever receives zero updates), so
 // each workgroup runs its own loop and reports a per-target status.
 //
-// Small m x m matrices live in a per-target global workspace (L2 / Infinity-Cache
-// resident), padded to mp = 16 ceil(m/16) with zero padding so every product runs as
-// whole v_mfma_f64_16x16x4f64 tiles. LDS holds what is latency-bound: the Jacobi
-// eigensolver's matrix (and its eigenvectors when they fit), the packed Cholesky
-// factor and its inverse, and reduction scratch. All reductions are deterministic.
+// Layout. Small m x m matrices live in a per-target global workspace (L2 / Infinity
+// Cache resident), padded to mp = 16 NT with zero padding. Every product is an FP64
+// MFMA GEMM whose operands are staged through LDS in 16-wide K panels (double
+// buffered); the 4 SIMDs of the CU each own one quadrant of the output tiles and the
+// two waves of a SIMD split that quadrant's tile rows, so every wave keeps a register
+// block of accumulators and each operand fragment feeds several MFMAs. The Cholesky
+// factor of I + trial^2 and its inverse stay in LDS as packed 16 x 16 lower blocks
+// (blocked right-looking factorisation; MFMA panel, trailing and inverse updates).
+// The eigendecomposition of the scaled Cr runs first, in its own kernel
+// (cyclic Jacobi with the matrix, and the eigenvectors when they fit, in LDS).
+// All reductions are deterministic.
+//
+// The CG iterates r, z, p, d and the gradient are exactly skew (built from skew()
+// outputs and from H = Z - Z^T, whose two halves are negated copies), so the
+// reference's preconditioner skew(R / den) equals R / den bit for bit; it is computed
+// that way.
 #include "common.h"
 
 // element-wise updates mirror the reference's explicitly rounded (non-contracted) ops
@@ -25,6 +36,7 @@ constexpr int DT = DW * WAVE;         // threads per target
 constexpr int MAX_NEWTON = 100, MAX_CG = 150, MAX_BACKTRACK = 40;   // dual_recommended_hvp.py:16
 constexpr double FEAS_TOL = 1e-8;                                    // dual_recommended_hvp.py:15
 constexpr int MAX_SWEEPS = 40;
+constexpr int NCOUNT = 12;            // counters per target
 
 // workspace slots, each mp*mp doubles
 enum Slot {
@@ -37,409 +49,778 @@ struct DualArgs {
     int m, mp;
     double* omega; long long sO_b; int ldo;      // m x m output (original basis)
     double* Pf; double* Qf;                       // [b][mp][mp] final P, Q (original basis)
-    long long* iters; int* status; int* counters; // counters [b][4]: newton, cg, backtracks, hvps
+    long long* iters; int* status; int* counters; // counters [b][12] (include/ssn.h)
     double* ws;                                   // [b][S_COUNT][mp*mp] + [b][mp] values
     int v_in_lds;
 };
 
+template <int NT>
+struct Cfg {
+    static constexpr int MP = 16 * NT;
+    static constexpr int KP = (NT % 2 == 0 && NT <= 6) ? 32 : 16;  // K panel width (divides MP)
+    static constexpr int AS = KP + 1;             // A panel row stride (doubles)
+    static constexpr int BS = MP + 2;             // B panel row stride
+    static constexpr int PANEL = MP * AS + KP * BS;
+    static constexpr int PER = 2 * KP * MP / DT;  // staged elements per thread and panel
+    static constexpr int DEPTH = KP == 32 ? 1 : 2;   // panels loaded ahead (a 32-wide panel
+                                                     // computes long enough to cover a load)
+    static constexpr int H = (NT + 1) / 2;        // quadrant split
+    static constexpr int MAXC = H;                // tile columns per wave
+    static constexpr int MAXR = (H + 1) / 2;      // tile rows per wave
+    static constexpr int NBLK = NT * (NT + 1) / 2;   // packed lower 16x16 blocks
+    // LDS layout (doubles): [0,48) scratch; two K panels; packed blocks; block-row temp; lambda
+    static constexpr int OFF_PANEL = 48;
+    static constexpr int OFF_BLK = OFF_PANEL + 2 * PANEL;
+    static constexpr int OFF_ROW = OFF_BLK + NBLK * 256;
+    static constexpr int OFF_LAM = OFF_ROW + (NT > 1 ? NT - 1 : 1) * 256;
+    static constexpr int LDS_END = OFF_LAM + MP;
+};
+
+// LDS scratch [0, 48): two alternating reduction buffers of 16 doubles, then flags.
+constexpr int RED_BUFS = 2, RED_SLOTS = 16, FLAG_JACOBI = 32, FLAG_CHOL = 33;
+
 struct Ctx {
-    int tid, wave, lane, r, q, m, mp, nt;
-    double* red;       // LDS [DW]
-    double* ws;        // this target's slots
-    __device__ double* slot(int s) const { return ws + (long long)s * mp * mp; }
+    int tid, wave, lane, r, q;
+    int m;
+    double* red;                 // LDS scratch
+    mutable int phase;           // which reduction buffer the next sum uses
 };
 
 __device__ __forceinline__ void sync() { __syncthreads(); }
 
-// Deterministic workgroup sum; every thread returns the same value.
-__device__ double wg_sum(const Ctx& c, double v) {
-    v = wave_sum(v);
-    if (c.lane == 0) c.red[c.wave] = v;
+// An opaque copy of a per-lane index. The engine inlines dozens of GEMM / element-wise
+// sites into one loop nest; without this, the compiler hoists every site's per-lane
+// address arithmetic out of the solver loops, keeps it all live and spills.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+__device__ __forceinline__ double readlane(double v, int lane) { return readlane_d(v, lane); }
+
+// Deterministic workgroup sums with one barrier: the two reduction buffers alternate,
+// and a wave cannot reuse a buffer before every wave has passed the barrier of the
+// sum in between. Every thread returns the same values.
+template <int WAVES>
+__device__ void block_sum2(const Ctx& c, double& a, double& b) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    double* buf = c.red + RED_SLOTS * c.phase;
+    c.phase ^= 1;
+    if (c.lane == 0) { buf[c.wave] = a; buf[8 + c.wave] = b; }
     sync();
-    double s = 0.0;
+    double s = 0.0, t = 0.0;
 #pragma unroll
-    for (int w = 0; w < DW; ++w) s += c.red[w];
-    sync();
-    return s;
+    for (int w = 0; w < WAVES; ++w) { s += buf[w]; t += buf[8 + w]; }
+    a = s; b = t;
 }
 
-// ---------------------------------------------------------------- MFMA tile products
-// acc += op(A) op(B) for one 16x16 tile (I, J), K = mp. A, B row-major mp x mp.
-template <bool TA, bool TB>
-__device__ __forceinline__ void tile_mac(const double* __restrict__ A, const double* __restrict__ B,
-                                         int mp, int I, int J, int r, int q, d4& acc)
-{
-    for (int k0 = 0; k0 < mp; k0 += 16) {
-        double a[4], b[4];
+__device__ __forceinline__ void wg_sum2(const Ctx& c, double& a, double& b) { block_sum2<DW>(c, a, b); }
+
+__device__ __forceinline__ double wg_sum(const Ctx& c, double v) {
+    double z = 0.0;
+    block_sum2<DW>(c, v, z);
+    return v;
+}
+
+// ---------------------------------------------------------------- tile ownership
+// SIMD s = wave & 3 owns quadrant (s >> 1, s & 1) of the NT x NT tile grid; the two
+// waves of a SIMD (w and w + 4) split the quadrant's tile rows.
+struct WT { int r0, nr, c0, nc; };
+
+template <int NT>
+__device__ __forceinline__ WT wave_tiles(int wave) {
+    constexpr int H = Cfg<NT>::H;
+    const int simd = wave & 3, half = wave >> 2;
+    const int qr0 = (simd >> 1) ? H : 0, qr1 = (simd >> 1) ? NT : H;
+    const int qc0 = (simd & 1) ? H : 0, qc1 = (simd & 1) ? NT : H;
+    const int rows = qr1 - qr0, split = (rows + 1) / 2;
+    WT w;
+    w.c0 = qc0; w.nc = qc1 - qc0;
+    w.r0 = half ? qr0 + split : qr0;
+    w.nr = half ? rows - split : split;
+    if (w.nr <= 0 || w.nc <= 0) { w.nr = 0; w.nc = 0; }
+    return w;
+}
+
+template <int NT>
+struct Acc {
+    d4 v[Cfg<NT>::MAXR][Cfg<NT>::MAXC];
+    __device__ __forceinline__ void zero() {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int k = k0 + 4 * u + q;
-            a[u] = TA ? A[k * mp + I * 16 + r] : A[(I * 16 + r) * mp + k];
-            b[u] = TB ? B[(J * 16 + r) * mp + k] : B[k * mp + J * 16 + r];
-        }
+        for (int i = 0; i < Cfg<NT>::MAXR; ++i)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc = mfma(a[u], b[u], acc);
+            for (int j = 0; j < Cfg<NT>::MAXC; ++j) v[i][j] = zero4();
     }
-}
+};
 
-__device__ __forceinline__ void tile_mac_neg(const double* __restrict__ A, const double* __restrict__ B,
-                                             int mp, int I, int J, int r, int q, d4& acc)
-{
-    // acc -= A B (negating the A operand is exact)
-    for (int k0 = 0; k0 < mp; k0 += 16) {
-        double a[4], b[4];
+// Epilogue over this wave's tiles: f(row, col, value); lane holds rows q + 4e, column r.
+template <int NT, class F>
+__device__ __forceinline__ void for_tiles(const Ctx& c, const WT& w, const Acc<NT>& acc, F f) {
+    const int q = opaque(c.q), r = opaque(c.r);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int k = k0 + 4 * u + q;
-            a[u] = -A[(I * 16 + r) * mp + k];
-            b[u] = B[k * mp + J * 16 + r];
-        }
+    for (int i = 0; i < Cfg<NT>::MAXR; ++i)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc = mfma(a[u], b[u], acc);
-    }
-}
-
-// C = op(A) op(B), with an epilogue f(row, col, value) per element.
-template <bool TA, bool TB, class F>
-__device__ void gemm(const Ctx& c, const double* A, const double* B, F f) {
-    const int nt = c.nt;
-    for (int t = c.wave; t < nt * nt; t += DW) {
-        const int I = t / nt, J = t - (t / nt) * nt;
-        d4 acc = zero4();
-        tile_mac<TA, TB>(A, B, c.mp, I, J, c.r, c.q, acc);
+        for (int j = 0; j < Cfg<NT>::MAXC; ++j)
+            if (i < w.nr && j < w.nc) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) f(I * 16 + c.q + 4 * i, J * 16 + c.r, acc[i]);
-    }
-}
-
-// C = A1 B1 - A2 B2 (all non-transposed), epilogue f.
-template <class F>
-__device__ void gemm_diff(const Ctx& c, const double* A1, const double* B1,
-                          const double* A2, const double* B2, F f) {
-    const int nt = c.nt;
-    for (int t = c.wave; t < nt * nt; t += DW) {
-        const int I = t / nt, J = t - (t / nt) * nt;
-        d4 acc = zero4();
-        tile_mac<false, false>(A1, B1, c.mp, I, J, c.r, c.q, acc);
-        tile_mac_neg(A2, B2, c.mp, I, J, c.r, c.q, acc);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) f(I * 16 + c.q + 4 * i, J * 16 + c.r, acc[i]);
-    }
-}
-
-// C = A1 B1^T + s A2 B2^T (s = +1 or -1), epilogue f.
-template <class F>
-__device__ void gemm_nt2(const Ctx& c, const double* A1, const double* B1,
-                         const double* A2, const double* B2, bool negate_second, F f) {
-    const int nt = c.nt, mp = c.mp;
-    for (int t = c.wave; t < nt * nt; t += DW) {
-        const int I = t / nt, J = t - (t / nt) * nt;
-        d4 acc = zero4();
-        tile_mac<false, true>(A1, B1, mp, I, J, c.r, c.q, acc);
-        for (int k0 = 0; k0 < mp; k0 += 16) {
-            double a[4], b[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int k = k0 + 4 * u + c.q;
-                const double av = A2[(I * 16 + c.r) * mp + k];
-                a[u] = negate_second ? -av : av;
-                b[u] = B2[(J * 16 + c.r) * mp + k];
+                for (int e = 0; e < 4; ++e)
+                    f(16 * (w.r0 + i) + q + 4 * e, 16 * (w.c0 + j) + r, acc.v[i][j][e]);
             }
+}
+
+// ---------------------------------------------------------------- staged GEMM
+// acc += sum over terms t of op(A_t) op(B_t), K = MP per term. Element sources:
+//   ga(t, i, k) -> op(A_t)[i][k],  gb(t, k, j) -> op(B_t)[k][j]  (global memory).
+// TA / TB say which index is contiguous in memory, for coalesced staging.
+template <int NT, bool TA, bool TB, class GA, class GB>
+__device__ void gemm_acc(const Ctx& c, double* lds, int nterms, GA ga, GB gb, Acc<NT>& acc, const WT& w) {
+    using C = Cfg<NT>;
+    constexpr int MP = C::MP, KP = C::KP, PER = C::PER;
+    constexpr int PPT = MP / KP;            // panels per term
+    double* buf0 = lds + C::OFF_PANEL;
+    constexpr int DEPTH = C::DEPTH;
+    double pre[DEPTH][PER];                 // panels in flight in registers
+    const int npanels = nterms * PPT;
+    // opaque copies of the lane indices keep the per-thread staging offsets of the many
+    // inlined GEMM sites from being hoisted out of the solver loops
+    const int tid = opaque(c.tid);
+    const int lr = opaque(c.r), lq = opaque(c.q);
+    auto load = [&](int p, double (&dst)[PER]) {
+        const int t = p / PPT, k0 = (p - t * PPT) * KP;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc = mfma(a[u], b[u], acc);
+        for (int s = 0; s < PER; ++s) {
+            const int e = tid + s * DT;
+            double v;
+            if (e < KP * MP) {               // A panel element
+                int i, kk;
+                if (TA) { kk = e / MP; i = e - kk * MP; } else { i = e / KP; kk = e - i * KP; }
+                v = ga(t, i, k0 + kk);
+            } else {                         // B panel element
+                const int f = e - KP * MP;
+                int kk, j;
+                if (TB) { j = f / KP; kk = f - j * KP; } else { kk = f / MP; j = f - kk * MP; }
+                v = gb(t, k0 + kk, j);
+            }
+            dst[s] = v;
         }
+    };
+    auto store = [&](const double (&src)[PER], double* dst) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) f(I * 16 + c.q + 4 * i, J * 16 + c.r, acc[i]);
+        for (int s = 0; s < PER; ++s) {
+            const int e = tid + s * DT;
+            if (e < KP * MP) {
+                int i, kk;
+                if (TA) { kk = e / MP; i = e - kk * MP; } else { i = e / KP; kk = e - i * KP; }
+                dst[i * C::AS + kk] = src[s];
+            } else {
+                const int f = e - KP * MP;
+                int kk, j;
+                if (TB) { j = f / KP; kk = f - j * KP; } else { kk = f / MP; j = f - kk * MP; }
+                dst[MP * C::AS + kk * C::BS + j] = src[s];
+            }
+        }
+    };
+    sync();    // prior writes of the operands (and prior panel reads) are complete
+#pragma unroll
+    for (int d = 0; d < DEPTH; ++d) if (d < npanels) load(d, pre[d]);
+#pragma unroll 1
+    for (int p = 0; p < npanels; ++p) {
+        // buffer (p & 1) was last read by panel p - 2, before every wave passed the
+        // barrier of panel p - 1
+        double* cur = buf0 + (p & 1) * C::PANEL;
+        const int slot = DEPTH == 1 ? 0 : (p & 1);
+        if (slot) store(pre[DEPTH - 1], cur); else store(pre[0], cur);
+        sync();
+        if (p + DEPTH < npanels) { if (slot) load(p + DEPTH, pre[DEPTH - 1]); else load(p + DEPTH, pre[0]); }
+        const double* As = cur;
+        const double* Bs = cur + MP * C::AS;
+#pragma unroll
+        for (int k = 0; k < KP; k += 4) {
+            double a[C::MAXR], b[C::MAXC];
+#pragma unroll
+            for (int i = 0; i < C::MAXR; ++i)
+                a[i] = (i < w.nr) ? As[(16 * (w.r0 + i) + lr) * C::AS + k + lq] : 0.0;
+#pragma unroll
+            for (int j = 0; j < C::MAXC; ++j)
+                b[j] = (j < w.nc) ? Bs[(k + lq) * C::BS + 16 * (w.c0 + j) + lr] : 0.0;
+#pragma unroll
+            for (int i = 0; i < C::MAXR; ++i)
+#pragma unroll
+                for (int j = 0; j < C::MAXC; ++j)
+                    if (i < w.nr && j < w.nc) acc.v[i][j] = mfma(a[i], b[j], acc.v[i][j]);
+        }
     }
 }
 
-// ---------------------------------------------------------------- element-wise helpers
-template <class F>
+// C = op(A) op(B) with global row-major MP x MP operands; epilogue f(row, col, value).
+template <int NT, bool TA, bool TB, class F>
+__device__ void gemm(const Ctx& c, double* lds, const double* A, const double* B, F f) {
+    constexpr int MP = Cfg<NT>::MP;
+    const WT w = wave_tiles<NT>(c.wave);
+    Acc<NT> acc; acc.zero();
+    gemm_acc<NT, TA, TB>(c, lds, 1,
+        [&](int, int i, int k) { return TA ? A[k * MP + i] : A[i * MP + k]; },
+        [&](int, int k, int j) { return TB ? B[j * MP + k] : B[k * MP + j]; }, acc, w);
+    for_tiles<NT>(c, w, acc, f);
+    sync();    // epilogue results are visible to every wave
+}
+
+// C = A1 B1 + s A2 B2 (non-transposed operands), epilogue f.
+template <int NT, class F>
+__device__ void gemm2(const Ctx& c, double* lds, const double* A1, const double* B1,
+                      const double* A2, const double* B2, double s, F f) {
+    constexpr int MP = Cfg<NT>::MP;
+    const WT w = wave_tiles<NT>(c.wave);
+    Acc<NT> acc; acc.zero();
+    gemm_acc<NT, false, false>(c, lds, 2,
+        [&](int t, int i, int k) { return t ? s * A2[i * MP + k] : A1[i * MP + k]; },
+        [&](int t, int k, int j) { return t ? B2[k * MP + j] : B1[k * MP + j]; }, acc, w);
+    for_tiles<NT>(c, w, acc, f);
+    sync();
+}
+
+// C = A1 B1^T + s A2 B2^T, epilogue f.
+template <int NT, class F>
+__device__ void gemm2_nt(const Ctx& c, double* lds, const double* A1, const double* B1,
+                         const double* A2, const double* B2, double s, F f) {
+    constexpr int MP = Cfg<NT>::MP;
+    const WT w = wave_tiles<NT>(c.wave);
+    Acc<NT> acc; acc.zero();
+    gemm_acc<NT, false, true>(c, lds, 2,
+        [&](int t, int i, int k) { return t ? s * A2[i * MP + k] : A1[i * MP + k]; },
+        [&](int t, int k, int j) { return t ? B2[j * MP + k] : B1[j * MP + k]; }, acc, w);
+    for_tiles<NT>(c, w, acc, f);
+    sync();
+}
+
+// ---------------------------------------------------------------- element-wise helper
+template <int NT, class F>
 __device__ __forceinline__ void each(const Ctx& c, F f) {
-    const int total = c.mp * c.mp;
-    for (int e = c.tid; e < total; e += DT) f(e / c.mp, e - (e / c.mp) * c.mp, e);
-}
-
-__device__ double dot(const Ctx& c, const double* X, const double* Y) {
-    double s = 0.0;
-    each(c, [&](int, int, int e) { s = fma(X[e], Y[e], s); });
-    return wg_sum(c, s);
-}
-
-// skew(R / (lambda_i + lambda_j)) computed at both positions (hvp_fused.cu:374-387).
-__device__ void precondition(const Ctx& c, const double* R, const double* lam, double* out, double sign) {
-    each(c, [&](int i, int j, int e) {
-        const double a = (sign * R[e]) / (lam[i] + lam[j]);
-        const double t = (sign * R[j * c.mp + i]) / (lam[j] + lam[i]);
-        out[e] = (a - t) * 0.5;
-    });
+    constexpr int MP = Cfg<NT>::MP;
+#pragma unroll 2
+    for (int e = opaque(c.tid); e < MP * MP; e += DT) f(e / MP, e % MP, e);
 }
 
 // ---------------------------------------------------------------- Jacobi eigensolver
-// Cyclic two-sided Jacobi with the round-robin (circle) ordering: each round applies
-// m/2 disjoint rotations in parallel. A (m x m, stride mp) is in LDS; V (stride mp) is
-// in LDS or global. Rotations are skipped when |a_pq| <= eps sqrt(|a_pp a_qq|), which
-// gives eigenvalues with backward error O(eps ||A||). Returns sweeps used (or -1).
-__device__ int jacobi(const Ctx& c, double* A, double* V, int* pairs, double* rot, int* flag) {
-    const int m = c.m, mp = c.mp, np = m / 2;
-    for (int e = c.tid; e < m * mp; e += DT) {
-        const int i = e / mp, j = e - (e / mp) * mp;
-        V[e] = (i == j) ? 1.0 : 0.0;
+// Cyclic two-sided Jacobi with the round-robin (circle) tournament: each round applies
+// m/2 disjoint rotations in parallel. A and V are kept in POSITION order: the pairs of
+// every round are the fixed slots (s, m-1-s), and the tournament's player rotation
+// (slot j -> j + 1 over 1..m-1, slot 0 fixed) is folded into the write-back of each
+// round. Every thread therefore owns fixed 2x2 blocks of A and fixed eigenvector
+// entries for the whole solve, and its LDS accesses run along rows (no bank conflicts).
+// A rotation is applied when |a_pq| > eps sqrt(|a_pp a_qq|); the iteration stops after
+// the first sweep at whose end no pair exceeds that threshold. On return the diagonal
+// of A holds the eigenvalues and the columns of V the eigenvectors, both in slot order.
+constexpr int JB_MAX = 5;     // 2x2 blocks per thread: ceil(64*65/2 / 512)
+constexpr int JV_MAX = 16;    // eigenvector pairs per thread: ceil(128*64 / 512)
+constexpr double JAC_EPS = 2.220446049250313e-16;
+
+__device__ __forceinline__ int slot_next(int j, int m) { return j == 0 ? 0 : (j == m - 1 ? 1 : j + 1); }
+
+template <int TH>
+__device__ int jacobi(const Ctx& c, double* A, int LA, double* V, int LV, double* rot, int* flag) {
+    static_assert(TH == 512, "work lists are sized for 512 threads");
+    const int m = c.m, np = m / 2;
+    // rotation parameters as structure of arrays: consecutive lanes read consecutive slots
+    double* rc = rot;
+    double* rs = rot + np;
+    double* rt = rot + 2 * np;
+    for (int e = c.tid; e < m * m; e += TH) {
+        const int i = e / m, j = e - (e / m) * m;
+        V[i * LV + j] = (i == j) ? 1.0 : 0.0;
     }
+    // fixed work lists: upper pair-blocks (P <= P2) and eigenvector items (row i, pair s)
+    int bP[JB_MAX], bQ[JB_MAX];
+#pragma unroll
+    for (int k = 0; k < JB_MAX; ++k) {
+        const int e = c.tid + k * TH;
+        bP[k] = -1; bQ[k] = 0;
+        if (e < np * (np + 1) / 2) {
+            int P = 0, start = 0, len = np;
+            while (e >= start + len) { start += len; ++P; --len; }
+            bP[k] = P; bQ[k] = P + (e - start);
+        }
+    }
+    int vI[JV_MAX], vS[JV_MAX];
+#pragma unroll
+    for (int k = 0; k < JV_MAX; ++k) {
+        const int e = c.tid + k * TH;
+        vI[k] = -1; vS[k] = 0;
+        if (e < m * np) { vI[k] = e / np; vS[k] = e - (e / np) * np; }
+    }
+    sync();
     for (int sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
-        if (c.tid == 0) *flag = 0;
-        sync();
+#pragma unroll 1
         for (int round = 0; round < m - 1; ++round) {
-            for (int s = c.tid; s < np; s += DT) {
-                // positions s and m-1-s; position 0 holds player 0, position j>0 holds
-                // player ((j - 1 + round) mod (m - 1)) + 1
-                const int pa = s == 0 ? 0 : ((s - 1 + round) % (m - 1)) + 1;
-                const int pb = ((m - 1 - s - 1 + round) % (m - 1)) + 1;
-                const int p = min(pa, pb), qq = max(pa, pb);
-                const double app = A[p * mp + p], aqq = A[qq * mp + qq], apq = A[p * mp + qq];
+            if (c.tid < np) {
+                const int p = c.tid, q = m - 1 - c.tid;
+                const double app = A[p * LA + p], aqq = A[q * LA + q], apq = A[p * LA + q];
                 double cs = 1.0, sn = 0.0, t = 0.0;
-                if (apq != 0.0 && fabs(apq) > 2.220446049250313e-16 * sqrt(fabs(app) * fabs(aqq))) {
+                if (apq != 0.0 && fabs(apq) > JAC_EPS * sqrt(fabs(app) * fabs(aqq))) {
                     const double theta = (aqq - app) / (2.0 * apq);
                     t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
                     if (theta < 0.0) t = -t;
                     cs = 1.0 / sqrt(t * t + 1.0);
                     sn = t * cs;
-                    *flag = 1;
                 }
-                pairs[2 * s] = p; pairs[2 * s + 1] = qq;
-                rot[3 * s] = cs; rot[3 * s + 1] = sn; rot[3 * s + 2] = t;
+                rc[p] = cs; rs[p] = sn; rt[p] = t;
             }
             sync();
-            // 2x2 blocks (P, P') with P <= P'; the owner writes the block and its mirror
-            for (int e = c.tid; e < np * np; e += DT) {
-                const int P = e / np, P2 = e - (e / np) * np;
-                if (P > P2) continue;
-                const int p = pairs[2 * P], qq = pairs[2 * P + 1];
-                const double cs = rot[3 * P], sn = rot[3 * P + 1];
+            // read and rotate
+            double za[JB_MAX][4], zv[JV_MAX][2];
+#pragma unroll
+            for (int k = 0; k < JB_MAX; ++k) {
+                if (bP[k] < 0) continue;
+                // opaque: keeps the per-item addresses from being hoisted out of the
+                // round loop (they would all stay live and spill)
+                const int P = opaque(bP[k]), P2 = opaque(bQ[k]), q = m - 1 - P, q2 = m - 1 - P2;
+                const double cs = rc[P], sn = rs[P];
                 if (P == P2) {
-                    const double t = rot[3 * P + 2];
-                    if (sn != 0.0) {
-                        const double apq = A[p * mp + qq];
-                        A[p * mp + p] -= t * apq;
-                        A[qq * mp + qq] += t * apq;
-                        A[p * mp + qq] = 0.0;
-                        A[qq * mp + p] = 0.0;
-                    }
-                    continue;
+                    const double t = rt[P], apq = A[P * LA + q];
+                    za[k][0] = A[P * LA + P] - t * apq;
+                    za[k][3] = A[q * LA + q] + t * apq;
+                    za[k][1] = (sn != 0.0) ? 0.0 : apq;
+                    za[k][2] = za[k][1];
+                } else {
+                    const double c2 = rc[P2], s2 = rs[P2];
+                    const double x00 = A[P * LA + P2], x01 = A[P * LA + q2];
+                    const double x10 = A[q * LA + P2], x11 = A[q * LA + q2];
+                    // right rotation (columns P2, q2), then left rotation (rows P, q)
+                    const double y00 = c2 * x00 - s2 * x01, y01 = s2 * x00 + c2 * x01;
+                    const double y10 = c2 * x10 - s2 * x11, y11 = s2 * x10 + c2 * x11;
+                    za[k][0] = cs * y00 - sn * y10; za[k][2] = sn * y00 + cs * y10;
+                    za[k][1] = cs * y01 - sn * y11; za[k][3] = sn * y01 + cs * y11;
                 }
-                const int p2 = pairs[2 * P2], q2 = pairs[2 * P2 + 1];
-                const double c2 = rot[3 * P2], s2 = rot[3 * P2 + 1];
-                if (sn == 0.0 && s2 == 0.0) continue;
-                const double x00 = A[p * mp + p2], x01 = A[p * mp + q2];
-                const double x10 = A[qq * mp + p2], x11 = A[qq * mp + q2];
-                // right rotation (columns p2, q2), then left rotation (rows p, q)
-                const double y00 = c2 * x00 - s2 * x01, y01 = s2 * x00 + c2 * x01;
-                const double y10 = c2 * x10 - s2 * x11, y11 = s2 * x10 + c2 * x11;
-                const double z00 = cs * y00 - sn * y10, z10 = sn * y00 + cs * y10;
-                const double z01 = cs * y01 - sn * y11, z11 = sn * y01 + cs * y11;
-                A[p * mp + p2] = z00; A[p * mp + q2] = z01;
-                A[qq * mp + p2] = z10; A[qq * mp + q2] = z11;
-                A[p2 * mp + p] = z00; A[q2 * mp + p] = z01;
-                A[p2 * mp + qq] = z10; A[q2 * mp + qq] = z11;
             }
-            for (int e = c.tid; e < m * np; e += DT) {
-                const int i = e / np, P = e - (e / np) * np;
-                const double sn = rot[3 * P + 1];
-                if (sn == 0.0) continue;
-                const double cs = rot[3 * P];
-                const int p = pairs[2 * P], qq = pairs[2 * P + 1];
-                const double vp = V[i * mp + p], vq = V[i * mp + qq];
-                V[i * mp + p] = cs * vp - sn * vq;
-                V[i * mp + qq] = sn * vp + cs * vq;
+#pragma unroll
+            for (int k = 0; k < JV_MAX; ++k) {
+                if (vI[k] < 0) continue;
+                const int s = opaque(vS[k]);
+                const double cs = rc[s], sn = rs[s];
+                const double* row = V + opaque(vI[k]) * LV;
+                const double vp = row[s], vq = row[m - 1 - s];
+                zv[k][0] = cs * vp - sn * vq;
+                zv[k][1] = sn * vp + cs * vq;
+            }
+            sync();
+            // write back into the next round's slots
+#pragma unroll
+            for (int k = 0; k < JB_MAX; ++k) {
+                if (bP[k] < 0) continue;
+                const int P = opaque(bP[k]), P2 = opaque(bQ[k]);
+                const int a0 = slot_next(P, m), a1 = slot_next(m - 1 - P, m);
+                const int b0 = slot_next(P2, m), b1 = slot_next(m - 1 - P2, m);
+                A[a0 * LA + b0] = za[k][0]; A[a0 * LA + b1] = za[k][1];
+                A[a1 * LA + b0] = za[k][2]; A[a1 * LA + b1] = za[k][3];
+                if (P != P2) {
+                    A[b0 * LA + a0] = za[k][0]; A[b1 * LA + a0] = za[k][1];
+                    A[b0 * LA + a1] = za[k][2]; A[b1 * LA + a1] = za[k][3];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < JV_MAX; ++k) {
+                if (vI[k] < 0) continue;
+                const int s = opaque(vS[k]);
+                double* row = V + opaque(vI[k]) * LV;
+                row[slot_next(s, m)] = zv[k][0];
+                row[slot_next(m - 1 - s, m)] = zv[k][1];
             }
             sync();
         }
-        const int any = *flag;
+        // convergence: does any pair still exceed the rotation threshold?
+        if (c.tid == 0) *flag = 0;
         sync();
-        if (!any) return sweep + 1;
+        int any = 0;
+        for (int e = c.tid; e < m * m; e += TH) {
+            const int i = e / m, j = e - (e / m) * m;
+            if (i < j) {
+                const double aij = A[i * LA + j];
+                if (aij != 0.0 && fabs(aij) > JAC_EPS * sqrt(fabs(A[i * LA + i]) * fabs(A[j * LA + j]))) any = 1;
+            }
+        }
+        if (any) *flag = 1;
+        sync();
+        const int more = *flag;
+        sync();
+        if (!more) return sweep + 1;
     }
     return -1;
 }
 
-// ---------------------------------------------------------------- Cholesky in LDS (packed)
-__device__ __forceinline__ int pk(int i, int j) { return i * (i + 1) / 2 + j; }   // j <= i
+// ---------------------------------------------------------------- blocked Cholesky / inverse
+// Packed lower 16x16 blocks in LDS: block (I, J), J <= I, at index I(I+1)/2 + J, row-major.
+__device__ __forceinline__ int bidx(int I, int J) { return I * (I + 1) / 2 + J; }
 
-// Lower Cholesky of K (global, stride mp, leading m x m block) into packed LDS L.
-// Returns false when a pivot is not positive (torch.linalg.cholesky_ex info != 0).
-__device__ bool cholesky(const Ctx& c, const double* K, double* L) {
-    const int m = c.m, mp = c.mp;
-    for (int e = c.tid; e < m * m; e += DT) {
-        const int i = e / m, j = e - (e / m) * m;
-        if (j <= i) L[pk(i, j)] = K[i * mp + j];
-    }
-    sync();
-    for (int j = 0; j < m; ++j) {
-        const double d = L[pk(j, j)];
-        if (!(d > 0.0)) { sync(); return false; }
-        const double ljj = sqrt(d);
-        const double inv = 1.0 / ljj;
-        sync();     // every thread has read the pivot before it is overwritten
-        for (int i = j + 1 + c.tid; i < m; i += DT) L[pk(i, j)] *= inv;
-        if (c.tid == 0) L[pk(j, j)] = ljj;
-        sync();
-        const int len = m - j - 1;
-        for (int e = c.tid; e < len * len; e += DT) {
-            const int i = j + 1 + e / len, k = j + 1 + (e - (e / len) * len);
-            if (k <= i) L[pk(i, k)] -= L[pk(i, j)] * L[pk(k, j)];
+// One wave: Cholesky of the 16x16 SPD block B (lower part read), replaced in place by
+// the inverse X of its factor (lower, zero above). Lane i < 16 holds row i in registers;
+// values move across lanes with readlane only. Each pivot takes one reciprocal square
+// root (hardware estimate plus two Newton steps) instead of a square root and a divide,
+// and the inverse is right-looking, so the dependent chains stay short. Returns false
+// on a non-positive pivot.
+__device__ __forceinline__ double rsqrt_refined(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    y = y * (1.5 - 0.5 * d * y * y);
+    y = y * (1.5 - 0.5 * d * y * y);
+    return y;
+}
+
+__device__ bool diag_factor_invert(double* B, int lane) {
+    double row[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) row[k] = (lane < 16) ? B[lane * 16 + k] : 0.0;
+    bool ok = true;
+    double rinv[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const double d = readlane(row[j], j);
+        if (!(d > 0.0)) ok = false;
+        const double y = rsqrt_refined(d);        // 1 / L_jj
+        rinv[j] = y;
+        if (lane == j) row[j] = d * y;            // L_jj
+        else if (lane > j) row[j] = row[j] * y;   // L_ij
+#pragma unroll
+        for (int k = j + 1; k < 16; ++k) {
+            const double lkj = readlane(row[j], k);
+            if (lane > j) row[k] = row[k] - row[j] * lkj;
         }
+    }
+    // column `lane` of X = L^-1, right-looking: x_k /= L_kk, then x_i -= L_ik x_k (i > k)
+    double x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = (lane == i) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        x[k] = x[k] * rinv[k];
+#pragma unroll
+        for (int i = k + 1; i < 16; ++i) x[i] = x[i] - readlane(row[k], i) * x[k];
+    }
+    if (lane < 16) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) B[i * 16 + lane] = x[i];
+    }
+    return ok;
+}
+
+// acc(D layout) -= L_IJ L_KJ^T for one 16x16 block pair, in place on block (I, K).
+__device__ __forceinline__ void trailing_block(double* Bk, const double* Li, const double* Lk, int r, int q) {
+    d4 acc;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] = Bk[(q + 4 * e) * 16 + r];
+    double a[4], b[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) { a[s] = -Li[r * 16 + 4 * s + q]; b[s] = Lk[r * 16 + 4 * s + q]; }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc = mfma(a[s], b[s], acc);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) Bk[(q + 4 * e) * 16 + r] = acc[e];
+}
+
+// Blocked right-looking Cholesky of the packed lower blocks with one-block lookahead
+// (wave 0 updates and factors diagonal block J+1 while the other waves finish the
+// trailing update of step J), then the blocked inverse R = L^-1 in place (diagonal
+// blocks already hold their inverses). Returns validity.
+template <int NT>
+__device__ bool cholesky_inverse(const Ctx& c, double* lds) {
+    using C = Cfg<NT>;
+    double* blk = lds + C::OFF_BLK;
+    double* tmp = lds + C::OFF_ROW;
+    int* okflag = reinterpret_cast<int*>(c.red + FLAG_CHOL);
+    const int r = opaque(c.r), q = opaque(c.q);
+    bool ok0 = true;                        // wave 0's running validity
+    if (c.wave == 0) ok0 = diag_factor_invert(blk, c.lane);
+    sync();
+#pragma unroll 1
+    for (int J = 0; J < NT; ++J) {
+        // panel: L_IJ = A_IJ X_JJ^T, I > J
+        const double* X = blk + bidx(J, J) * 256;
+        for (int I = J + 1 + c.wave; I < NT; I += DW) {
+            double* B = blk + bidx(I, J) * 256;
+            d4 acc = zero4();
+            double a[4], b[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) { a[s] = B[r * 16 + 4 * s + q]; b[s] = X[r * 16 + 4 * s + q]; }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) acc = mfma(a[s], b[s], acc);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) B[(q + 4 * e) * 16 + r] = acc[e];
+        }
+        sync();
+        if (J + 1 == NT) break;
+        // trailing update A_IK -= L_IJ L_KJ^T, J < K <= I; block (J+1, J+1) goes to wave 0,
+        // which then factors it right away; the other waves share the remaining blocks
+        if (c.wave == 0) {
+            const int I = J + 1;
+            trailing_block(blk + bidx(I, I) * 256, blk + bidx(I, J) * 256, blk + bidx(I, J) * 256, r, q);
+            ok0 = diag_factor_invert(blk + bidx(I, I) * 256, c.lane) && ok0;
+        } else {
+            const int rem = NT - 1 - J;
+            const int count = rem * (rem + 1) / 2;          // block 0 of the list is (J+1, J+1)
+            for (int t = c.wave; t < count; t += DW - 1) {
+                int I = J + 1, base = 0;
+                while (t >= base + (I - J)) { base += I - J; ++I; }
+                const int K = J + 1 + (t - base);
+                trailing_block(blk + bidx(I, K) * 256, blk + bidx(I, J) * 256, blk + bidx(K, J) * 256, r, q);
+            }
+        }
+        sync();
+    }
+    if (c.wave == 0 && c.lane == 0) *okflag = ok0 ? 1 : 0;
+    sync();
+    const bool valid = *okflag != 0;
+    if (!valid) return false;
+    // inverse: R_IJ = -X_II sum_{K=J}^{I-1} L_IK R_KJ, block rows top to bottom
+#pragma unroll 1
+    for (int I = 1; I < NT; ++I) {
+        const int J = c.wave;
+        if (J < I) {
+            d4 S = zero4();
+            for (int K = J; K < I; ++K) {
+                const double* Lik = blk + bidx(I, K) * 256;
+                const double* Rkj = blk + bidx(K, J) * 256;
+                double a[4], b[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) { a[s] = Lik[r * 16 + 4 * s + q]; b[s] = Rkj[(4 * s + q) * 16 + r]; }
+#pragma unroll
+                for (int s = 0; s < 4; ++s) S = mfma(a[s], b[s], S);
+            }
+            const double* Xi = blk + bidx(I, I) * 256;
+            d4 acc = zero4();
+#pragma unroll
+            for (int s = 0; s < 4; ++s) acc = mfma(-Xi[r * 16 + 4 * s + q], S[s], acc);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) tmp[J * 256 + (q + 4 * e) * 16 + r] = acc[e];
+        }
+        sync();
+        for (int e = c.tid; e < I * 256; e += DT) blk[bidx(I, 0) * 256 + e] = tmp[e];
         sync();
     }
     return true;
 }
 
-// R = L^{-1} (packed lower, LDS) by right-looking forward substitution, then written to
-// global Rg (stride mp, zero elsewhere in the mp x mp block).
-__device__ void tri_inverse(const Ctx& c, const double* L, double* Rp, double* Rg) {
-    const int m = c.m, mp = c.mp;
-    for (int e = c.tid; e < m * (m + 1) / 2; e += DT) Rp[e] = 0.0;
-    sync();
-    for (int e = c.tid; e < m; e += DT) Rp[pk(e, e)] = 1.0;
-    sync();
-    // B := I; for i: X[i][0..i] = B[i][0..i] / L[i][i]; B[k][j] -= L[k][i] X[i][j] (k > i, j <= i)
-    for (int i = 0; i < m; ++i) {
-        const double inv = 1.0 / L[pk(i, i)];
-        for (int j = c.tid; j <= i; j += DT) Rp[pk(i, j)] *= inv;
-        sync();
-        const int rows = m - i - 1, cols = i + 1;
-        for (int e = c.tid; e < rows * cols; e += DT) {
-            const int k = i + 1 + e / cols, j = e - (e / cols) * cols;
-            Rp[pk(k, j)] -= L[pk(k, i)] * Rp[pk(i, j)];
+// P = R^T R from the packed inverse factor (upper tiles computed, mirrored), written to
+// global P (row-major MP x MP) with the padding rows / columns set to zero.
+template <int NT>
+__device__ void inverse_product(const Ctx& c, const double* lds, double* P) {
+    using C = Cfg<NT>;
+    constexpr int MP = C::MP;
+    const double* blk = lds + C::OFF_BLK;
+    const int r = opaque(c.r), q = opaque(c.q), m = c.m;
+    for (int t = c.wave; t < C::NBLK; t += DW) {
+        int I = 0, J = 0, base = 0;
+        while (t >= base + (NT - I)) { base += NT - I; ++I; }
+        J = I + (t - base);                       // I <= J
+        d4 acc = zero4();
+        for (int K = J; K < NT; ++K) {
+            const double* Rki = blk + bidx(K, I) * 256;
+            const double* Rkj = blk + bidx(K, J) * 256;
+            double a[4], b[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) { a[s] = Rki[(4 * s + q) * 16 + r]; b[s] = Rkj[(4 * s + q) * 16 + r]; }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) acc = mfma(a[s], b[s], acc);
         }
-        sync();
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int i = 16 * I + q + 4 * e, j = 16 * J + r;
+            const double v = (i < m && j < m) ? acc[e] : 0.0;
+            P[i * MP + j] = v;
+            if (I != J) P[j * MP + i] = v;
+        }
     }
-    each(c, [&](int i, int j, int e) { Rg[e] = (i < m && j <= i) ? Rp[pk(i, j)] : 0.0; });
     sync();
 }
 
-// ---------------------------------------------------------------- the engine
-__global__ __launch_bounds__(DT)
-void dual_kernel(DualArgs a)
+// ---------------------------------------------------------------- eigensolver kernel
+// Per target (one 8-wave workgroup): scale = max(1, tr(Cr)/m) (dual_recommended_hvp.py:106), eigh(Cr / scale)
+// (:108) by Jacobi, eigenvalues sorted ascending. Writes the sorted eigenvalues (padding
+// 1.0, which keeps lambda_i + lambda_j > 0), the eigenvectors (zero padded, slot S_V),
+// scale, the sweep count and the elapsed ticks into the target's workspace.
+constexpr int ET = 512;               // threads of the eigensolver kernel
+constexpr int EW = ET / WAVE;
+
+struct EighArgs {
+    const double* Cr; long long sC_b; int ldc;
+    int m, mp;
+    double* ws;
+    int v_in_lds;
+};
+
+__host__ __device__ __forceinline__ long long target_doubles(int mp) {
+    return (long long)S_COUNT * mp * mp + mp + 16;
+}
+
+__global__ __launch_bounds__(ET)
+void eigh_kernel(EighArgs a)
 {
     extern __shared__ double lds[];
     Ctx c;
-    c.tid = threadIdx.x; c.wave = c.tid >> 6; c.lane = c.tid & 63;
-    c.r = c.lane & 15; c.q = c.lane >> 4;
-    c.m = a.m; c.mp = a.mp; c.nt = a.mp / 16;
-    const int b = blockIdx.x;
-    const int m = c.m, mp = c.mp;
-    const long long mm = (long long)mp * mp;
-    c.red = lds;
-    int* flag = reinterpret_cast<int*>(lds + DW);
-    double* big = lds + 16;
-    c.ws = a.ws + (long long)b * (S_COUNT * mm + mp);
-    double* lam = c.ws + S_COUNT * mm;
-
-    double* V = c.slot(S_V);    double* Cie = c.slot(S_CIE); double* Jse = c.slot(S_JSE);
-    double* W = c.slot(S_W);    double* P = c.slot(S_P);     double* Q = c.slot(S_Q);
-    double* Sr = c.slot(S_SR);  double* Si = c.slot(S_SI);   double* G = c.slot(S_G);
-    double* Dd = c.slot(S_D);   double* R = c.slot(S_R);     double* Z0 = c.slot(S_Z0);
-    double* Pp = c.slot(S_PP);  double* Hp = c.slot(S_HP);   double* U = c.slot(S_U);
-    double* Vv = c.slot(S_VV);  double* T1 = c.slot(S_T1);   double* T2 = c.slot(S_T2);
-    double* T3 = c.slot(S_T3);  double* T4 = c.slot(S_T4);   double* T5 = c.slot(S_T5);
-
+    c.tid = threadIdx.x; c.wave = __builtin_amdgcn_readfirstlane(c.tid >> 6); c.lane = c.tid & 63;
+    c.r = c.lane & 15; c.q = c.lane >> 4; c.m = a.m; c.red = lds; c.phase = 0;
+    const int b = blockIdx.x, m = a.m, mp = a.mp;
+    const unsigned long long t0 = wall_clock64();
+    double* ws = a.ws + (long long)b * target_doubles(mp);
+    double* Vout = ws + (long long)S_V * mp * mp;
+    double* lam = ws + (long long)S_COUNT * mp * mp;
+    double* extra = lam + mp;
     const double* Crb = a.Cr + (long long)b * a.sC_b;
-    const double* Cib = a.Ci + (long long)b * a.sC_b;
 
-    // 1. scale = max(1, tr(Cr)/m); Cr/scale into LDS, Ci/scale into T1 (dual_recommended_hvp.py:106-107)
-    double tr = 0.0;
-    for (int i = c.tid; i < m; i += DT) tr += Crb[(long long)i * a.ldc + i];
-    tr = wg_sum(c, tr);
-    const double scale = fmax(tr / m, 1.0);
-    double* Aj = big;                                   // Jacobi matrix, stride mp
-    int* pairs = reinterpret_cast<int*>(big + m * mp);
-    double* rot = big + m * mp + m;                     // 3 * (m/2) doubles after the pair ints
-    double* Vj = a.v_in_lds ? (rot + 3 * (m / 2) + 2) : T5;
-    each(c, [&](int i, int j, int e) {
-        const bool in = i < m && j < m;
-        // lower triangle, as eigh(UPLO='L') reads it
-        if (i < m) Aj[i * mp + j] = in ? Crb[(long long)max(i, j) * a.ldc + min(i, j)] / scale : 0.0;
-        T1[e] = in ? Cib[(long long)i * a.ldc + j] / scale : 0.0;
-    });
+    const int LA = m + 1;     // odd row stride: the column-wise mirror writes hit distinct banks
+    double* Aj = lds + 48;
+    double* Vj = a.v_in_lds ? Aj + m * LA : ws + (long long)S_T5 * mp * mp;
+    const int LV = a.v_in_lds ? LA : mp;
+    double* tail = a.v_in_lds ? Vj + m * LA : Aj + m * LA;
+    int* pairs = reinterpret_cast<int*>(tail);            // m ints: ranks after the sweeps
+    double* rot = tail + m;                               // 3 m / 2 doubles
+    int* flag = reinterpret_cast<int*>(c.red + FLAG_JACOBI);
+
+    if (c.wave == 0) {
+        double t = 0.0;
+        for (int i = c.lane; i < m; i += WAVE) t += Crb[(long long)i * a.ldc + i];
+        t = wave_sum(t);
+        if (c.lane == 0) c.red[40] = t;
+    }
     sync();
-
-    // 2. eigh(Cr / scale) (dual_recommended_hvp.py:108) by Jacobi, sorted ascending
-    const int sweeps = jacobi(c, Aj, Vj, pairs, rot, flag);
-    for (int i = c.tid; i < mp; i += DT) {
-        if (i >= m) { lam[i] = 1.0; continue; }       // padding: keeps lambda_i + lambda_j > 0
-        const double li = Aj[i * mp + i];
+    const double scale = fmax(c.red[40] / m, 1.0);
+    for (int e = c.tid; e < m * m; e += ET) {
+        const int i = e / m, j = e - (e / m) * m;
+        // lower triangle, as eigh(UPLO='L') reads it
+        Aj[i * LA + j] = Crb[(long long)max(i, j) * a.ldc + min(i, j)] / scale;
+    }
+    sync();
+    const int sweeps = jacobi<ET>(c, Aj, LA, Vj, LV, rot, flag);
+    for (int i = c.tid; i < m; i += ET) {
+        const double li = Aj[i * LA + i];
         int rank = 0;
         for (int j = 0; j < m; ++j) {
-            const double lj = Aj[j * mp + j];
+            const double lj = Aj[j * LA + j];
             rank += (lj < li || (lj == li && j < i)) ? 1 : 0;
         }
         pairs[i] = rank;     // pairs[] is free after the sweeps
     }
     sync();
-    for (int i = c.tid; i < m; i += DT) lam[pairs[i]] = Aj[i * mp + i];
-    each(c, [&](int i, int j, int e) {
-        // V[:, rank(j)] = Vj[:, j]; zero padding
-        if (i < m && j < m) V[i * mp + pairs[j]] = Vj[i * mp + j];
-        else V[e] = 0.0;
+    for (int i = c.tid; i < mp; i += ET) {
+        if (i < m) lam[pairs[i]] = Aj[i * LA + i];
+        else lam[i] = 1.0;
+    }
+    for (int e = c.tid; e < mp * mp; e += ET) {
+        const int i = e / mp, j = e - (e / mp) * mp;
+        if (i < m && j < m) Vout[i * mp + pairs[j]] = Vj[i * LV + j];
+        else Vout[e] = 0.0;
+    }
+    if (c.tid == 0) {
+        extra[0] = scale;
+        extra[1] = (double)sweeps;
+        extra[2] = (double)(wall_clock64() - t0);
+        extra[3] = (double)a.v_in_lds;
+    }
+}
+
+// ---------------------------------------------------------------- the engine
+template <int NT>
+__global__ __launch_bounds__(DT)
+void dual_kernel(DualArgs a)
+{
+    using Cf = Cfg<NT>;
+    constexpr int MP = Cf::MP;
+    constexpr long long MM = (long long)MP * MP;
+    extern __shared__ double lds[];
+    Ctx c;
+    c.tid = threadIdx.x; c.wave = __builtin_amdgcn_readfirstlane(c.tid >> 6); c.lane = c.tid & 63;
+    c.r = c.lane & 15; c.q = c.lane >> 4;
+    c.m = a.m;
+    c.red = lds;
+    c.phase = 0;
+    const int b = blockIdx.x;
+    const int m = c.m;
+    const unsigned long long t_start = wall_clock64();
+    unsigned long long t_cg = 0, t_ls = 0, t_setup = 0;
+
+    double* ws = a.ws + (long long)b * target_doubles(MP);
+    auto slot = [&](int s) { return ws + s * MM; };
+    double* lam = ws + S_COUNT * MM;
+    const double* extra = lam + MP;
+    double* V = slot(S_V);    double* Cie = slot(S_CIE); double* Jse = slot(S_JSE);
+    double* W = slot(S_W);    double* P = slot(S_P);     double* Q = slot(S_Q);
+    double* Sr = slot(S_SR);  double* Si = slot(S_SI);   double* G = slot(S_G);
+    double* Dd = slot(S_D);   double* R = slot(S_R);     double* Z0 = slot(S_Z0);
+    double* Pp = slot(S_PP);  double* Hp = slot(S_HP);   double* U = slot(S_U);
+    double* Vv = slot(S_VV);  double* T1 = slot(S_T1);   double* T2 = slot(S_T2);
+    double* T3 = slot(S_T3);  double* T4 = slot(S_T4);   double* T5 = slot(S_T5);
+    double* lam_s = lds + Cf::OFF_LAM;
+
+    const double* Cib = a.Ci + (long long)b * a.sC_b;
+    const double scale = extra[0];
+    const int sweeps = (int)extra[1];
+    const unsigned long long t_eigh = (unsigned long long)extra[2];
+    const int v_in_lds = (int)extra[3];
+    for (int i = c.tid; i < MP; i += DT) lam_s[i] = lam[i];
+    // Ci / scale into T1 (dual_recommended_hvp.py:107)
+    each<NT>(c, [&](int i, int j, int e) {
+        T1[e] = (i < m && j < m) ? Cib[(long long)i * a.ldc + j] / scale : 0.0;
     });
     sync();
 
     // 3. eigenbasis: Ci <- skew(V^T Ci V), J <- skew(V^T J(V)) / scale (:110-111, :53-55)
-    gemm<false, false>(c, T1, V, [&](int i, int j, double v) { T2[i * mp + j] = v; });
-    each(c, [&](int i, int j, int e) {
-        // J(V) = [V_bot; -V_top] over the leading m rows
-        const int k = m / 2;
-        T3[e] = (i < k) ? V[(i + k) * mp + j] : (i < m ? -V[(i - k) * mp + j] : 0.0);
+    gemm<NT, false, false>(c, lds, T1, V, [&](int i, int j, double v) { T2[i * MP + j] = v; });
+    each<NT>(c, [&](int i, int j, int e) {
+        const int k = m / 2;      // J(V) = [V_bot; -V_top] over the leading m rows
+        T3[e] = (i < k) ? V[(i + k) * MP + j] : (i < m ? -V[(i - k) * MP + j] : 0.0);
     });
     sync();
-    gemm<true, false>(c, V, T2, [&](int i, int j, double v) { T4[i * mp + j] = v; });
-    gemm<true, false>(c, V, T3, [&](int i, int j, double v) { T1[i * mp + j] = v; });
-    sync();
-    each(c, [&](int i, int j, int e) {
-        Cie[e] = (T4[e] - T4[j * mp + i]) / 2;
-        Jse[e] = ((T1[e] - T1[j * mp + i]) / 2) / scale;
+    gemm<NT, true, false>(c, lds, V, T2, [&](int i, int j, double v) { T4[i * MP + j] = v; });
+    gemm<NT, true, false>(c, lds, V, T3, [&](int i, int j, double v) { T1[i * MP + j] = v; });
+    each<NT>(c, [&](int i, int j, int e) {
+        Cie[e] = (T4[e] - T4[j * MP + i]) / 2;
+        Jse[e] = ((T1[e] - T1[j * MP + i]) / 2) / scale;
         W[e] = 0.0;
         P[e] = (i == j && i < m) ? 1.0 : 0.0;
         Q[e] = 0.0;
     });
     sync();
+    t_setup = wall_clock64() - t_start;
 
     long long iters = 0;
     int status = sweeps < 0 ? ST_EIGH : ST_OK;
     int n_cg = 0, n_bt = 0, n_hvp = 0;
     const double sqrt_m = sqrt((double)m);
-    double* Lp = big;                              // packed Cholesky factor
-    double* Rp = big + m * (m + 1) / 2;            // packed inverse factor
 
+#pragma unroll 1
     for (int ns = 0; ns < MAX_NEWTON && status == ST_OK; ++ns) {
-        // -- sandwich S = N C N^H (:120-125)
+        // -- sandwich S = N C N^H and the gradient J - Si (:120-129)
+        double gg = 0.0;
         if (ns == 0) {
-            each(c, [&](int i, int j, int e) { Sr[e] = (i == j) ? (i < m ? lam[i] : 0.0) : 0.0; Si[e] = Cie[e]; });
+            each<NT>(c, [&](int i, int j, int e) {
+                Sr[e] = (i == j && i < m) ? lam_s[i] : 0.0;
+                const double si = Cie[e];
+                Si[e] = si;
+                const double g = Jse[e] - si;
+                G[e] = g;
+                gg = fma(g, g, gg);
+            });
         } else {
-            gemm<false, false>(c, W, P, [&](int i, int j, double v) { Q[i * mp + j] = -v; });
-            sync();
+            gemm<NT, false, false>(c, lds, W, P, [&](int i, int j, double v) { Q[i * MP + j] = -v; });
             // Yr = P diag(lam) - Q Ci -> T1 ; Yi = P Ci + Q diag(lam) -> T2 (hvp_fused.cu:39-80)
-            gemm<false, false>(c, Q, Cie, [&](int i, int j, double v) {
-                const double d = j < m ? lam[j] : 0.0;
-                T1[i * mp + j] = P[i * mp + j] * d - v;
+            gemm<NT, false, false>(c, lds, Q, Cie, [&](int i, int j, double v) {
+                const double d = j < m ? lam_s[j] : 0.0;
+                T1[i * MP + j] = P[i * MP + j] * d - v;
             });
-            gemm<false, false>(c, P, Cie, [&](int i, int j, double v) {
-                const double d = j < m ? lam[j] : 0.0;
-                T2[i * mp + j] = v + Q[i * mp + j] * d;
+            gemm<NT, false, false>(c, lds, P, Cie, [&](int i, int j, double v) {
+                const double d = j < m ? lam_s[j] : 0.0;
+                T2[i * MP + j] = v + Q[i * MP + j] * d;
             });
-            sync();
             // Ur = Yr P^T + Yi Q^T -> T3 ; Ui = Yi P^T - Yr Q^T -> T4 (recovery_fused.cu on m x m)
-            gemm_nt2(c, T1, P, T2, Q, false, [&](int i, int j, double v) { T3[i * mp + j] = v; });
-            gemm_nt2(c, T2, P, T1, Q, true, [&](int i, int j, double v) { T4[i * mp + j] = v; });
-            sync();
-            each(c, [&](int i, int j, int e) {
-                Sr[e] = (T3[e] + T3[j * mp + i]) / 2;
-                Si[e] = (T4[e] - T4[j * mp + i]) / 2;
+            gemm2_nt<NT>(c, lds, T1, P, T2, Q, 1.0, [&](int i, int j, double v) { T3[i * MP + j] = v; });
+            gemm2_nt<NT>(c, lds, T2, P, T1, Q, -1.0, [&](int i, int j, double v) { T4[i * MP + j] = v; });
+            each<NT>(c, [&](int i, int j, int e) {
+                Sr[e] = (T3[e] + T3[j * MP + i]) / 2;
+                const double si = (T4[e] - T4[j * MP + i]) / 2;
+                Si[e] = si;
+                const double g = Jse[e] - si;
+                G[e] = g;
+                gg = fma(g, g, gg);
             });
         }
-        sync();
-        // -- gradient and residual (:126-129)
-        double gg = 0.0;
-        each(c, [&](int, int, int e) { const double g = Jse[e] - Si[e]; G[e] = g; gg = fma(g, g, gg); });
         const double gnorm = sqrt(wg_sum(c, gg));
         const double residual = gnorm * scale / sqrt_m;
         if (!(residual >= FEAS_TOL / 2)) {
@@ -448,94 +829,108 @@ void dual_kernel(DualArgs a)
         }
         ++iters;
 
-        // -- direction (:138-162)
+        // -- direction (:138-162); precondition(R) = R / (lam_i + lam_j) for skew R
         if (ns == 0) {
-            precondition(c, G, lam, Dd, -1.0);
+            each<NT>(c, [&](int i, int j, int e) {
+                const double z = (-G[e]) / (lam_s[i] + lam_s[j]);
+                Dd[e] = z; Z0[e] = z;
+            });
             sync();
-            each(c, [&](int, int, int e) { Z0[e] = Dd[e]; });
         } else {
-            precondition(c, G, lam, Z0, -1.0);
-            each(c, [&](int, int, int e) { R[e] = -G[e]; Dd[e] = 0.0; });
-            sync();
-            double rz = dot(c, R, Z0);
-            each(c, [&](int, int, int e) { Pp[e] = Z0[e]; });
+            double rz = 0.0, dummy = 0.0;
+            each<NT>(c, [&](int i, int j, int e) {
+                const double rv = -G[e];
+                const double z = rv / (lam_s[i] + lam_s[j]);
+                R[e] = rv; Dd[e] = 0.0; Z0[e] = z; Pp[e] = z;
+                rz = fma(rv, z, rz);
+            });
+            wg_sum2(c, rz, dummy);
             const double tol = fmin(fmax(sqrt(residual), 1e-8), 0.1) * gnorm;
-            sync();
+            const unsigned long long t0 = wall_clock64();
+#pragma unroll 1
             for (int it = 0; it < MAX_CG; ++it) {
                 // Hp = hvp(P, Q, Sr, Si, p): U = p Sr, V = p Si, Z = P U - Q V, H = Z - Z^T
-                gemm<false, false>(c, Pp, Sr, [&](int i, int j, double v) { U[i * mp + j] = v; });
-                gemm<false, false>(c, Pp, Si, [&](int i, int j, double v) { Vv[i * mp + j] = v; });
-                sync();
-                gemm_diff(c, P, U, Q, Vv, [&](int i, int j, double v) { T1[i * mp + j] = v; });
-                sync();
-                each(c, [&](int i, int j, int e) { Hp[e] = (i == j) ? 0.0 : T1[e] - T1[j * mp + i]; });
-                sync();
+                gemm<NT, false, false>(c, lds, Pp, Sr, [&](int i, int j, double v) { U[i * MP + j] = v; });
+                gemm<NT, false, false>(c, lds, Pp, Si, [&](int i, int j, double v) { Vv[i * MP + j] = v; });
+                gemm2<NT>(c, lds, P, U, Q, Vv, -1.0, [&](int i, int j, double v) { T1[i * MP + j] = v; });
                 ++n_hvp; ++n_cg;
-                const double curvature = dot(c, Pp, Hp);
+                double curv = 0.0;
+                each<NT>(c, [&](int i, int j, int e) {
+                    const double h = (i == j) ? 0.0 : T1[e] - T1[j * MP + i];
+                    Hp[e] = h;
+                    curv = fma(Pp[e], h, curv);
+                });
+                const double curvature = wg_sum(c, curv);
                 if (!(curvature > 0.0)) break;
                 const double alpha = rz / curvature;
-                double rr = 0.0;
-                each(c, [&](int, int, int e) {
+                double rr = 0.0, rzn = 0.0;
+                each<NT>(c, [&](int i, int j, int e) {
                     Dd[e] = Dd[e] + alpha * Pp[e];
                     const double rv = R[e] - alpha * Hp[e];
                     R[e] = rv;
+                    const double z = rv / (lam_s[i] + lam_s[j]);
+                    T2[e] = z;
                     rr = fma(rv, rv, rr);
+                    rzn = fma(rv, z, rzn);
                 });
-                const double rnorm = sqrt(wg_sum(c, rr));
-                if (!(rnorm > tol)) break;
-                precondition(c, R, lam, T2, 1.0);
-                sync();
-                const double next_rz = dot(c, R, T2);
-                const double beta = next_rz / rz;
-                each(c, [&](int, int, int e) { Pp[e] = T2[e] + beta * Pp[e]; });
-                rz = next_rz;
+                wg_sum2(c, rr, rzn);
+                if (!(sqrt(rr) > tol)) break;
+                const double beta = rzn / rz;
+                each<NT>(c, [&](int, int, int e) { Pp[e] = T2[e] + beta * Pp[e]; });
+                rz = rzn;
                 sync();
             }
-        }
-        sync();
-        // -- fallback to the preconditioned gradient (:163-164)
-        double slope = dot(c, G, Dd);
-        if (slope >= 0.0) {
-            each(c, [&](int, int, int e) { Dd[e] = Z0[e]; });
             sync();
-            slope = dot(c, G, Dd);
+            t_cg += wall_clock64() - t0;
+        }
+        // -- fallback to the preconditioned gradient (:163-164)
+        double slope = 0.0, dummy2 = 0.0;
+        each<NT>(c, [&](int, int, int e) { slope = fma(G[e], Dd[e], slope); });
+        wg_sum2(c, slope, dummy2);
+        if (slope >= 0.0) {
+            double s2 = 0.0, d3 = 0.0;
+            each<NT>(c, [&](int, int, int e) { Dd[e] = Z0[e]; s2 = fma(G[e], Z0[e], s2); });
+            wg_sum2(c, s2, d3);
+            slope = s2;
         }
 
         // -- Armijo backtracking with the exact change (:166-188)
         double step = 1.0;
         bool accepted = false;
+        const unsigned long long t1 = wall_clock64();
+#pragma unroll 1
         for (int bt = 0; bt < MAX_BACKTRACK; ++bt) {
             ++n_bt;
-            each(c, [&](int, int, int e) { T3[e] = step * Dd[e]; });                // delta
-            sync();
-            each(c, [&](int i, int j, int e) {
-                T1[e] = ((W[e] + T3[e]) - (W[j * mp + i] + T3[j * mp + i])) / 2;   // trial
+            double first = 0.0;
+            each<NT>(c, [&](int i, int j, int e) {
+                const double de = step * Dd[e];
+                const double dt = step * Dd[j * MP + i];
+                T3[e] = de;                                              // delta
+                T1[e] = ((W[e] + de) - (W[j * MP + i] + dt)) / 2;        // trial = skew(omega + delta)
+                first = fma(G[e], de, first);
             });
             sync();
-            gemm<false, false>(c, T1, T1, [&](int i, int j, double v) {
-                T2[i * mp + j] = (i == j ? 1.0 : 0.0) + v;                          // K = I + trial^2
+            // K = I + trial^2 into the packed lower blocks
+            double* blk = lds + Cf::OFF_BLK;
+            gemm<NT, false, false>(c, lds, T1, T1, [&](int i, int j, double v) {
+                const int I = i >> 4, J = j >> 4;
+                if (J <= I) blk[bidx(I, J) * 256 + (i & 15) * 16 + (j & 15)] = (i == j ? 1.0 : 0.0) + v;
             });
             sync();
-            const bool valid = cholesky(c, T2, Lp);
+            const bool valid = cholesky_inverse<NT>(c, lds);
             if (valid) {
-                tri_inverse(c, Lp, Rp, T4);                                          // R = L^-1
-                gemm<true, false>(c, T4, T4, [&](int i, int j, double v) { T2[i * mp + j] = v; });  // P' = R^T R
-                gemm<false, false>(c, T3, Sr, [&](int i, int j, double v) { U[i * mp + j] = v; });  // delta Sr
-                gemm<false, false>(c, T3, Si, [&](int i, int j, double v) { Vv[i * mp + j] = v; }); // delta Si
-                sync();
-                gemm<false, false>(c, T2, T3, [&](int i, int j, double v) { T4[i * mp + j] = v; }); // P' delta
-                gemm<false, false>(c, T1, Vv, [&](int i, int j, double v) { T5[i * mp + j] = v; }); // trial delta Si
-                sync();
-                double tail = 0.0, first = 0.0;
-                each(c, [&](int, int, int e) {
-                    tail = fma(T4[e], U[e] + T5[e], tail);
-                    first = fma(G[e], T3[e], first);
+                inverse_product<NT>(c, lds, T2);                        // P' = (L L^T)^-1
+                gemm<NT, false, false>(c, lds, T3, Sr, [&](int i, int j, double v) { U[i * MP + j] = v; });
+                gemm<NT, false, false>(c, lds, T3, Si, [&](int i, int j, double v) { Vv[i * MP + j] = v; });
+                gemm<NT, false, false>(c, lds, T2, T3, [&](int i, int j, double v) { T4[i * MP + j] = v; });
+                double tail = 0.0;
+                gemm<NT, false, false>(c, lds, T1, Vv, [&](int i, int j, double v) {
+                    tail = fma(T4[i * MP + j], U[i * MP + j] + v, tail);   // V (delta Sr + trial delta Si)
                 });
-                tail = wg_sum(c, tail);
-                first = wg_sum(c, first);
+                wg_sum2(c, tail, first);
                 const double change = first + tail;
                 if (change <= 1e-4 * step * slope) {
-                    each(c, [&](int, int, int e) { W[e] = T1[e]; P[e] = T2[e]; });
+                    each<NT>(c, [&](int, int, int e) { W[e] = T1[e]; P[e] = T2[e]; });
                     sync();
                     accepted = true;
                     break;
@@ -543,52 +938,72 @@ void dual_kernel(DualArgs a)
             }
             step = step / 2;
         }
+        t_ls += wall_clock64() - t1;
         if (!accepted) { status = ST_NO_DESCENT; break; }
         if (ns == MAX_NEWTON - 1) status = ST_NEWTON_CAP;
     }
     sync();
 
     // 4. back to the original basis (:58-64): omega = skew(V W V^T), P, Q rotated
-    gemm<false, true>(c, W, V, [&](int i, int j, double v) { T1[i * mp + j] = v; });
-    gemm<false, true>(c, P, V, [&](int i, int j, double v) { T2[i * mp + j] = v; });
-    gemm<false, true>(c, Q, V, [&](int i, int j, double v) { T3[i * mp + j] = v; });
-    sync();
-    double* Pf = a.Pf + (long long)b * mm;
-    double* Qf = a.Qf + (long long)b * mm;
-    gemm<false, false>(c, V, T1, [&](int i, int j, double v) { T4[i * mp + j] = v; });
-    gemm<false, false>(c, V, T2, [&](int i, int j, double v) { Pf[i * mp + j] = v; });
-    gemm<false, false>(c, V, T3, [&](int i, int j, double v) { Qf[i * mp + j] = v; });
-    sync();
+    gemm<NT, false, true>(c, lds, W, V, [&](int i, int j, double v) { T1[i * MP + j] = v; });
+    gemm<NT, false, true>(c, lds, P, V, [&](int i, int j, double v) { T2[i * MP + j] = v; });
+    gemm<NT, false, true>(c, lds, Q, V, [&](int i, int j, double v) { T3[i * MP + j] = v; });
+    double* Pf = a.Pf + (long long)b * MM;
+    double* Qf = a.Qf + (long long)b * MM;
+    gemm<NT, false, false>(c, lds, V, T1, [&](int i, int j, double v) { T4[i * MP + j] = v; });
+    gemm<NT, false, false>(c, lds, V, T2, [&](int i, int j, double v) { Pf[i * MP + j] = v; });
+    gemm<NT, false, false>(c, lds, V, T3, [&](int i, int j, double v) { Qf[i * MP + j] = v; });
     double* Ob = a.omega + (long long)b * a.sO_b;
     for (int e = c.tid; e < m * m; e += DT) {
         const int i = e / m, j = e - (e / m) * m;
-        Ob[(long long)i * a.ldo + j] = (T4[i * mp + j] - T4[j * mp + i]) / 2;
+        Ob[(long long)i * a.ldo + j] = (T4[i * MP + j] - T4[j * MP + i]) / 2;
     }
     if (c.tid == 0) {
         a.iters[b] = iters;
         a.status[b] = status;
         if (a.counters) {
-            a.counters[4 * b + 0] = (int)iters;
-            a.counters[4 * b + 1] = n_cg;
-            a.counters[4 * b + 2] = n_bt;
-            a.counters[4 * b + 3] = n_hvp;
+            int* cnt = a.counters + NCOUNT * b;
+            cnt[0] = (int)iters; cnt[1] = n_cg; cnt[2] = n_bt; cnt[3] = n_hvp;
+            cnt[4] = sweeps; cnt[5] = v_in_lds;
+            cnt[6] = (int)t_eigh; cnt[7] = (int)t_setup; cnt[8] = (int)t_cg; cnt[9] = (int)t_ls;
+            cnt[10] = (int)(wall_clock64() - t_start); cnt[11] = 0;
         }
     }
 }
 
-// ---------------------------------------------------------------- launcher
+// ---------------------------------------------------------------- launchers
 size_t dual_workspace_doubles(int batch, int m) {
-    const long long mp = pad16(m);
-    return (size_t)batch * (S_COUNT * mp * mp + mp);
+    return (size_t)batch * (size_t)target_doubles(pad16(m));
 }
 
-static size_t dual_lds_bytes(int m, bool v_in_lds) {
-    const int mp = pad16(m);
-    const size_t head = 16 * sizeof(double);
-    const size_t jac = (size_t)m * mp * 8 + (size_t)m * 8 + 3 * (m / 2) * 8 + 16 +
-                       (v_in_lds ? (size_t)m * mp * 8 : 0);
-    const size_t chol = (size_t)m * (m + 1) * 8;
-    return head + (jac > chol ? jac : chol);
+static size_t eigh_lds_doubles(int m, bool v_in_lds) {
+    return 48 + (size_t)m * (m + 1) * (v_in_lds ? 2 : 1) + 3 * (size_t)m + 8;
+}
+
+static hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int ldc, int batch,
+                              int m, double* ws, size_t lds_limit) {
+    const bool v_in_lds = eigh_lds_doubles(m, true) * 8 <= lds_limit;
+    const size_t lds = eigh_lds_doubles(m, v_in_lds) * 8;
+    if (lds > lds_limit) return hipErrorInvalidValue;
+    EighArgs e;
+    e.Cr = Cr; e.sC_b = sC_b; e.ldc = ldc; e.m = m; e.mp = pad16(m); e.ws = ws;
+    e.v_in_lds = v_in_lds ? 1 : 0;
+    hipError_t err = hipFuncSetAttribute((const void*)eigh_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(eigh_kernel, dim3(batch), dim3(ET), lds, stream, e);
+    return hipGetLastError();
+}
+
+template <int NT>
+static hipError_t launch_nt(hipStream_t stream, DualArgs a, int batch, size_t lds_limit) {
+    const size_t lds = (size_t)Cfg<NT>::LDS_END * 8;
+    if (lds > lds_limit) return hipErrorInvalidValue;
+    hipError_t err = hipFuncSetAttribute((const void*)dual_kernel<NT>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(dual_kernel<NT>, dim3(batch), dim3(DT), lds, stream, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_dual(hipStream_t stream, const double* Cr, const double* Ci, long long sC_b, int ldc,
@@ -597,19 +1012,23 @@ hipError_t launch_dual(hipStream_t stream, const double* Cr, const double* Ci, l
                        double* ws, size_t lds_limit)
 {
     const int mp = pad16(m);
-    bool v_in_lds = dual_lds_bytes(m, true) <= lds_limit;
-    const size_t lds = dual_lds_bytes(m, v_in_lds);
-    if (lds > lds_limit) return hipErrorInvalidValue;
+    hipError_t err = launch_eigh(stream, Cr, sC_b, ldc, batch, m, ws, lds_limit);
+    if (err != hipSuccess) return err;
     DualArgs a;
     a.Cr = Cr; a.Ci = Ci; a.sC_b = sC_b; a.ldc = ldc; a.m = m; a.mp = mp;
     a.omega = omega; a.sO_b = sO_b; a.ldo = ldo; a.Pf = Pf; a.Qf = Qf;
-    a.iters = iters; a.status = status; a.counters = counters; a.ws = ws;
-    a.v_in_lds = v_in_lds ? 1 : 0;
-    hipError_t err = hipFuncSetAttribute((const void*)dual_kernel,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(dual_kernel, dim3(batch), dim3(DT), lds, stream, a);
-    return hipGetLastError();
+    a.iters = iters; a.status = status; a.counters = counters; a.ws = ws; a.v_in_lds = 0;
+    switch (mp / 16) {
+        case 1: return launch_nt<1>(stream, a, batch, lds_limit);
+        case 2: return launch_nt<2>(stream, a, batch, lds_limit);
+        case 3: return launch_nt<3>(stream, a, batch, lds_limit);
+        case 4: return launch_nt<4>(stream, a, batch, lds_limit);
+        case 5: return launch_nt<5>(stream, a, batch, lds_limit);
+        case 6: return launch_nt<6>(stream, a, batch, lds_limit);
+        case 7: return launch_nt<7>(stream, a, batch, lds_limit);
+        case 8: return launch_nt<8>(stream, a, batch, lds_limit);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace ssn

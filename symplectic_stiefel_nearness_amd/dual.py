@@ -82,8 +82,9 @@ def _workspace(nbytes, device):
 def solve_device(T, return_stats=False, check=True):
     """Device-resident solve: T is a float64 HIP tensor [batch, 2n, 2k]; outputs stay on the device.
 
-    Returns (X, omega, iterations[, stats]) where stats = dict(status, counters[batch, 4]
-    with columns newton, cg, backtracks, hvps).
+    Returns (X, omega, iterations[, stats]) where stats = dict(status, counters[batch, 12]
+    with columns newton, cg, backtracks, hvps, Jacobi sweeps, 0, then 10 ns device ticks of
+    eigensolver, setup, CG, line search, whole kernel, 0).
     """
     lib = _require_ready()
     batch, n, m = _check_targets(T)
@@ -95,7 +96,7 @@ def solve_device(T, return_stats=False, check=True):
     omega = torch.empty((batch, m, m), dtype=torch.float64, device=dev)
     iters = torch.zeros(batch, dtype=torch.int64, device=dev)
     status = torch.zeros(batch, dtype=torch.int32, device=dev)
-    counters = torch.zeros((batch, 4), dtype=torch.int32, device=dev)
+    counters = torch.zeros((batch, 12), dtype=torch.int32, device=dev)
     if batch:
         nbytes = lib.ssn_workspace_bytes(batch, n, m)
         ws = _workspace(nbytes, dev)
@@ -179,7 +180,7 @@ def dual_batch(Cr, Ci, check=True):
     Q = torch.empty_like(P)
     iters = torch.zeros(batch, dtype=torch.int64, device=dev)
     status = torch.zeros(batch, dtype=torch.int32, device=dev)
-    counters = torch.zeros((batch, 4), dtype=torch.int32, device=dev)
+    counters = torch.zeros((batch, 12), dtype=torch.int32, device=dev)
     if batch:
         ws = _workspace(lib.ssn_workspace_bytes(batch, 1, m), dev)
         rc = lib.ssn_dual(_stream(), Cr.data_ptr(), Ci.data_ptr(), Cr.stride(0), Cr.stride(1), batch, m,

@@ -31,7 +31,10 @@ def say(*args):
 
 def test_device_properties(ssn):
     props = torch.cuda.get_device_properties(0)
-    say("device", props.name, "CUs", props.multi_processor_count, "gcnArch", getattr(props, "gcnArchName", "?"))
+    say("device", props.name, "CUs", props.multi_processor_count, "gcnArch", getattr(props, "gcnArchName", "?"),
+        "LDS/block", getattr(props, "shared_memory_per_block", "?"),
+        "LDS/block optin", getattr(props, "shared_memory_per_block_optin", "?"),
+        "LDS/CU", getattr(props, "shared_memory_per_multiprocessor", "?"))
     lib = ssn.dual._lib.load()
     say("ssn_version", lib.ssn_version().decode())
     assert lib.ssn_init() == 0
