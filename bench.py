@@ -84,38 +84,37 @@ def main():
     m = 2 * K
     log(f"inputs ready: {BATCH} x {tuple(T.shape[1:])}")
 
-    def step():
+    def step(ev=None):
+        if ev:
+            ev[0].record(stream)
         Cr, Ci = ssn.gram_chunks(T)
-        omega, P, Q, iters, st = ssn.dual_batch(Cr, Ci, check=False)
+        omega, P, Q, iters, st = ssn.dual_batch_staged(Cr, Ci, events=ev[1:4] if ev else None, check=False)
         X = ssn.recover_matmul(T, P, Q)
+        if ev:
+            ev[4].record(stream)
         return X, omega, iters, st
 
+    stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     log("warm-up done")
 
-    # timed region: K steps, HIP events on the stream every kernel is launched on
-    stream = torch.cuda.current_stream()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    # timed region: K steps; HIP events on the stream every kernel is launched on
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in range(args.steps):
-        e = ev[s]
-        e[0].record(stream)
-        Cr, Ci = ssn.gram_chunks(T)
-        e[1].record(stream)
-        omega, P, Q, iters, st = ssn.dual_batch(Cr, Ci, check=False)
-        e[2].record(stream)
-        X = ssn.recover_matmul(T, P, Q)
-        e[3].record(stream)
+    for s_ in range(args.steps):
+        X, omega, iters, st = step(ev[s_])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gram_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps
-    dual_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps
-    rec_ms = sum(e[2].elapsed_time(e[3]) for e in ev) / args.steps
+    eigh_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps
+    newton_ms = sum(e[2].elapsed_time(e[3]) for e in ev) / args.steps
+    rec_ms = sum(e[3].elapsed_time(e[4]) for e in ev) / args.steps
     ms_per_step = wall * 1e3 / args.steps
-    log(f"batched: {ms_per_step:.3f} ms/step (gram {gram_ms:.3f}, dual {dual_ms:.3f}, recover {rec_ms:.3f})")
+    log(f"batched: {ms_per_step:.3f} ms/step (gram {gram_ms:.3f}, eigh {eigh_ms:.3f}, "
+        f"newton {newton_ms:.3f}, recover {rec_ms:.3f})")
 
     status = st["status"].cpu()
     counters = st["counters"].cpu()
@@ -134,17 +133,19 @@ def main():
     fused_ms = (time.perf_counter() - t1) * 1e3 / 3
 
     # plain: per-target loop (experiment.py:75-80), device-resident
-    plain_ms = []
-    for rep in range(args.plain_reps + 1):
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        for j in range(BATCH):
-            ssn.solve_device(T[j:j + 1], check=False)
-        torch.cuda.synchronize()
-        if rep:
-            plain_ms.append((time.perf_counter() - t2) * 1e3)
-    plain_ms = sum(plain_ms) / len(plain_ms)
-    log(f"plain per-target loop: {plain_ms:.2f} ms for {BATCH} targets")
+    plain_ms = None
+    if args.plain_reps > 0:
+        runs = []
+        for rep in range(args.plain_reps + 1):
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            for j in range(BATCH):
+                ssn.solve_device(T[j:j + 1], check=False)
+            torch.cuda.synchronize()
+            if rep:
+                runs.append((time.perf_counter() - t2) * 1e3)
+        plain_ms = sum(runs) / len(runs)
+        log(f"plain per-target loop: {plain_ms:.2f} ms for {BATCH} targets")
 
     # reference protocol boundary: CPU tensor in -> CPU tensors out (PCIe included)
     ssn.solve(T_cpu)
@@ -156,27 +157,24 @@ def main():
     log(f"CPU->CPU batched solve: {pcie_ms:.2f} ms")
 
     fd = dual_flops(m, counters)
-    f_stream = 16.0 * N * m * m * BATCH
+    f_eigh = 9.0 * m ** 3 * BATCH               # dense symmetric eigensolver with vectors
+    f_gram = 8.0 * N * m * m * BATCH
+    f_rec = 8.0 * N * m * m * BATCH
+    f_stream = f_gram + f_rec
     bytes_fwd = 48.0 * N * m * BATCH
-    stages = {"gram": gram_ms, "dual": dual_ms, "recover": rec_ms}
-    dominant = max(stages, key=stages.get)
-    if dominant == "dual":
-        achieved = fd / (dual_ms * 1e-3) / 1e12
-        roof = {"kernel": "ssn::dual_kernel", "bound": "mfma", "achieved": achieved,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": None, "algorithmic_flops_per_launch": fd}
-    elif dominant == "gram":
-        f = 8.0 * N * m * m * BATCH
-        achieved = f / (gram_ms * 1e-3) / 1e12
-        roof = {"kernel": "ssn::gram_partial_kernel", "bound": "mfma", "achieved": achieved,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": None, "algorithmic_flops_per_launch": f}
-    else:
-        f = 8.0 * N * m * m * BATCH
-        achieved = f / (rec_ms * 1e-3) / 1e12
-        roof = {"kernel": "ssn::recover_kernel", "bound": "mfma", "achieved": achieved,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": None, "algorithmic_flops_per_launch": f}
+
+    def roof(kernel, flops, ms):
+        achieved = flops / (ms * 1e-3) / 1e12
+        return {"kernel": kernel, "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "algorithmic_flops_per_launch": flops, "avg_ms": ms}
+
+    kernels = [roof("ssn::gram_partial_kernel (+ gram_reduce_kernel)", f_gram, gram_ms),
+               roof("ssn::eigh_kernel", f_eigh, eigh_ms),
+               roof("ssn::dual_kernel", fd, newton_ms),
+               roof("ssn::recover_kernel (+ build_recovery_fragments)", f_rec, rec_ms)]
+    dominant = max(kernels, key=lambda k: k["avg_ms"])
+    stages = {"gram": gram_ms, "eigh": eigh_ms, "newton": newton_ms, "recover": rec_ms}
 
     out = {
         "metric": METRIC,
@@ -193,19 +191,21 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain FP64 Dual, batched, 64 targets, n=4096, k=48 (T 8192x96), "
                                "device-resident", "batch": BATCH, "n": N, "k": K},
-        "plain_solves_per_s": BATCH / (plain_ms * 1e-3),
+        "plain_solves_per_s": BATCH / (plain_ms * 1e-3) if plain_ms else None,
         "single_call_solves_per_s": BATCH / (fused_ms * 1e-3),
         "pcie_inclusive_solves_per_s": BATCH / (pcie_ms * 1e-3),
         "fwd_bwd_ms": None,
         "stage_ms": stages,
         "streaming_tflops": f_stream / ((gram_ms + rec_ms) * 1e-3) / 1e12,
+        "dual_phase_ms": eigh_ms + newton_ms,
         "forward_bytes_per_step": bytes_fwd,
         "dual_flops_per_step": fd,
         "iterations": {"newton_mean": float(counters[:, 0].double().mean()),
                        "cg_mean": float(counters[:, 1].double().mean()),
                        "backtracks_mean": float(counters[:, 2].double().mean())},
         "acceptance": acc,
-        "roofline": roof,
+        "roofline": dominant,
+        "roofline_kernels": kernels,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

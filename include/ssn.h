@@ -72,6 +72,17 @@ int ssn_dual(void* stream, const double* Cr, const double* Ci, long long sC_b, i
              double* omega, long long sO_b, int ldo, double* P, double* Q,
              long long* iters, int* status, int* counters, void* ws, size_t ws_bytes);
 
+/* The two halves of ssn_dual, for callers that time or overlap them separately: ssn_dual_eigh
+ * computes scale = max(1, tr(Cr)/m) and the eigendecomposition of Cr / scale
+ * (dual_recommended_hvp.py:106-108, torch.linalg.eigh) into `ws`; ssn_dual_newton then runs
+ * the Newton / PCG / Armijo iteration from that `ws` (same outputs as ssn_dual). The same ws
+ * (at least ssn_workspace_bytes(batch, 1, m)) must be passed to both, in that order. */
+int ssn_dual_eigh(void* stream, const double* Cr, long long sC_b, int ldc, int batch, int m,
+                  void* ws, size_t ws_bytes);
+int ssn_dual_newton(void* stream, const double* Cr, const double* Ci, long long sC_b, int ldc, int batch, int m,
+                    double* omega, long long sO_b, int ldo, double* P, double* Q,
+                    long long* iters, int* status, int* counters, void* ws, size_t ws_bytes);
+
 /* X = [A P^T + D Q^T ; D P^T - A Q^T] with P, Q [batch][mp][mp] from ssn_dual.
  * X: [batch][2n][m], strides sX_b, sX_r.
  * Replaces recover_block / recover_fused and recover_matmul

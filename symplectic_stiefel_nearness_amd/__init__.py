@@ -10,7 +10,7 @@ The arithmetic runs in libssn.so (hand-written HIP for gfx950) through a C ABI
 (include/ssn.h); this package is the host-side mirror of the reference interface.
 """
 from .dual import (initialize_kernels, solve, solve_device, gram_chunks, dual_batch,
-                   recover_matmul, FEAS_TOL, MAX_NEWTON, MAX_CG, MAX_BACKTRACK)
+                   dual_batch_staged, recover_matmul, FEAS_TOL, MAX_NEWTON, MAX_CG, MAX_BACKTRACK)
 
 __all__ = ["initialize_kernels", "solve", "solve_device", "gram_chunks", "dual_batch",
-           "recover_matmul"]
+           "dual_batch_staged", "recover_matmul"]

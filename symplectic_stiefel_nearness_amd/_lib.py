@@ -34,6 +34,8 @@ SIGNATURES = {
     "ssn_workspace_bytes": (_S, [_I, _I, _I]),
     "ssn_gram": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _P, _L, _I, _P, _S]),
     "ssn_dual": (_I, [_P, _P, _P, _L, _I, _I, _I, _P, _L, _I, _P, _P, _P, _P, _P, _P, _S]),
+    "ssn_dual_eigh": (_I, [_P, _P, _L, _I, _I, _I, _P, _S]),
+    "ssn_dual_newton": (_I, [_P, _P, _P, _L, _I, _I, _I, _P, _L, _I, _P, _P, _P, _P, _P, _P, _S]),
     "ssn_recover": (_I, [_P, _P, _L, _L, _P, _P, _I, _I, _I, _P, _L, _L, _P, _S]),
     "ssn_solve": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _L, _I, _P, _P, _P, _P, _S]),
 }

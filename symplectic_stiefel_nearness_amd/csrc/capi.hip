@@ -16,6 +16,9 @@ hipError_t launch_recover(hipStream_t, const double*, long long, long long, cons
 size_t dual_workspace_doubles(int batch, int m);
 hipError_t launch_dual(hipStream_t, const double*, const double*, long long, int, int, int,
                        double*, long long, int, double*, double*, long long*, int*, int*, double*, size_t);
+hipError_t launch_dual_eigh(hipStream_t, const double*, long long, int, int, int, double*, size_t);
+hipError_t launch_dual_newton(hipStream_t, const double*, const double*, long long, int, int, int,
+                              double*, long long, int, double*, double*, long long*, int*, int*, double*, size_t);
 }  // namespace ssn
 
 using namespace ssn;
@@ -138,6 +141,33 @@ int ssn_dual(void* stream, const double* Cr, const double* Ci, long long sC_b, i
     if (!ws || ws_bytes < dual_workspace_doubles(batch, m) * sizeof(double)) return SSN_ERR_WORKSPACE;
     return hip_status(launch_dual((hipStream_t)stream, Cr, Ci, sC_b, ldc, batch, m, omega, sO_b, ldo,
                                   P, Q, iters, status, counters, (double*)ws, lds));
+}
+
+int ssn_dual_eigh(void* stream, const double* Cr, long long sC_b, int ldc, int batch, int m,
+                  void* ws, size_t ws_bytes)
+{
+    if (!shape_ok(batch, 1, m) || !Cr || ldc < m) return SSN_ERR_ARGS;
+    if (batch == 0) return SSN_OK;
+    size_t lds = 0;
+    int rc = current_device_ready(&lds);
+    if (rc) return rc;
+    if (!ws || ws_bytes < dual_workspace_doubles(batch, m) * sizeof(double)) return SSN_ERR_WORKSPACE;
+    return hip_status(launch_dual_eigh((hipStream_t)stream, Cr, sC_b, ldc, batch, m, (double*)ws, lds));
+}
+
+int ssn_dual_newton(void* stream, const double* Cr, const double* Ci, long long sC_b, int ldc, int batch, int m,
+                    double* omega, long long sO_b, int ldo, double* P, double* Q,
+                    long long* iters, int* status, int* counters, void* ws, size_t ws_bytes)
+{
+    if (!shape_ok(batch, 1, m) || !Cr || !Ci || !omega || !P || !Q || !iters || !status ||
+        ldc < m || ldo < m) return SSN_ERR_ARGS;
+    if (batch == 0) return SSN_OK;
+    size_t lds = 0;
+    int rc = current_device_ready(&lds);
+    if (rc) return rc;
+    if (!ws || ws_bytes < dual_workspace_doubles(batch, m) * sizeof(double)) return SSN_ERR_WORKSPACE;
+    return hip_status(launch_dual_newton((hipStream_t)stream, Cr, Ci, sC_b, ldc, batch, m, omega, sO_b, ldo,
+                                         P, Q, iters, status, counters, (double*)ws, lds));
 }
 
 int ssn_recover(void* stream, const double* T, long long sT_b, long long sT_r, const double* P,

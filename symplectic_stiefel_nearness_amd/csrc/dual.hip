@@ -1006,14 +1006,18 @@ static hipError_t launch_nt(hipStream_t stream, DualArgs a, int batch, size_t ld
     return hipGetLastError();
 }
 
-hipError_t launch_dual(hipStream_t stream, const double* Cr, const double* Ci, long long sC_b, int ldc,
-                       int batch, int m, double* omega, long long sO_b, int ldo,
-                       double* Pf, double* Qf, long long* iters, int* status, int* counters,
-                       double* ws, size_t lds_limit)
+hipError_t launch_dual_eigh(hipStream_t stream, const double* Cr, long long sC_b, int ldc, int batch, int m,
+                            double* ws, size_t lds_limit)
+{
+    return launch_eigh(stream, Cr, sC_b, ldc, batch, m, ws, lds_limit);
+}
+
+hipError_t launch_dual_newton(hipStream_t stream, const double* Cr, const double* Ci, long long sC_b, int ldc,
+                              int batch, int m, double* omega, long long sO_b, int ldo,
+                              double* Pf, double* Qf, long long* iters, int* status, int* counters,
+                              double* ws, size_t lds_limit)
 {
     const int mp = pad16(m);
-    hipError_t err = launch_eigh(stream, Cr, sC_b, ldc, batch, m, ws, lds_limit);
-    if (err != hipSuccess) return err;
     DualArgs a;
     a.Cr = Cr; a.Ci = Ci; a.sC_b = sC_b; a.ldc = ldc; a.m = m; a.mp = mp;
     a.omega = omega; a.sO_b = sO_b; a.ldo = ldo; a.Pf = Pf; a.Qf = Qf;
@@ -1029,6 +1033,17 @@ hipError_t launch_dual(hipStream_t stream, const double* Cr, const double* Ci, l
         case 8: return launch_nt<8>(stream, a, batch, lds_limit);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_dual(hipStream_t stream, const double* Cr, const double* Ci, long long sC_b, int ldc,
+                       int batch, int m, double* omega, long long sO_b, int ldo,
+                       double* Pf, double* Qf, long long* iters, int* status, int* counters,
+                       double* ws, size_t lds_limit)
+{
+    hipError_t err = launch_eigh(stream, Cr, sC_b, ldc, batch, m, ws, lds_limit);
+    if (err != hipSuccess) return err;
+    return launch_dual_newton(stream, Cr, Ci, sC_b, ldc, batch, m, omega, sO_b, ldo, Pf, Qf,
+                              iters, status, counters, ws, lds_limit);
 }
 
 }  // namespace ssn

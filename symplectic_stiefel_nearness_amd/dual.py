@@ -194,6 +194,50 @@ def dual_batch(Cr, Ci, check=True):
 
 
 @torch.no_grad()
+def dual_batch_staged(Cr, Ci, events=None, check=True):
+    """dual_batch as its two launches, ssn_dual_eigh then ssn_dual_newton, on one workspace.
+
+    `events`, when given, is three torch.cuda.Event recorded before the eigensolver, between
+    the two launches and after the Newton iteration (for per-kernel timing).
+    """
+    lib = _require_ready()
+    assert Cr.shape == Ci.shape and Cr.ndim == 3 and Cr.shape[-1] == Cr.shape[-2]
+    assert Cr.dtype == Ci.dtype == torch.float64 and Cr.is_cuda and Ci.is_cuda
+    batch, m, _ = Cr.shape
+    assert m % 2 == 0 and 2 <= m <= MAX_ORDER
+    Cr = Cr.contiguous()
+    Ci = Ci.contiguous()
+    mp = lib.ssn_padded_order(m)
+    dev = Cr.device
+    omega = torch.empty((batch, m, m), dtype=torch.float64, device=dev)
+    P = torch.empty((batch, mp, mp), dtype=torch.float64, device=dev)
+    Q = torch.empty_like(P)
+    iters = torch.zeros(batch, dtype=torch.int64, device=dev)
+    status = torch.zeros(batch, dtype=torch.int32, device=dev)
+    counters = torch.zeros((batch, 12), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    if batch:
+        ws = _workspace(lib.ssn_workspace_bytes(batch, 1, m), dev)
+        if events:
+            events[0].record(stream)
+        rc = lib.ssn_dual_eigh(_stream(), Cr.data_ptr(), Cr.stride(0), Cr.stride(1), batch, m,
+                               ws.data_ptr(), ws.numel())
+        _lib.check(rc, "ssn_dual_eigh")
+        if events:
+            events[1].record(stream)
+        rc = lib.ssn_dual_newton(_stream(), Cr.data_ptr(), Ci.data_ptr(), Cr.stride(0), Cr.stride(1), batch, m,
+                                 omega.data_ptr(), omega.stride(0), omega.stride(1), P.data_ptr(), Q.data_ptr(),
+                                 iters.data_ptr(), status.data_ptr(), counters.data_ptr(),
+                                 ws.data_ptr(), ws.numel())
+        _lib.check(rc, "ssn_dual_newton")
+        if events:
+            events[2].record(stream)
+    if check:
+        _raise_for_status(status)
+    return omega, P, Q, iters, dict(status=status, counters=counters)
+
+
+@torch.no_grad()
 def recover_matmul(T, P, Q):
     """X = [A P^T + D Q^T; D P^T - A Q^T] on device (dual_recommended_hvp.py:194-228)."""
     lib = _require_ready()
