@@ -98,6 +98,18 @@ int ssn_solve(void* stream, const double* T, long long sT_b, long long sT_r, int
               double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
               long long* iters, int* status, int* counters, void* ws, size_t ws_bytes);
 
+/* Backward of the projection (no counterpart in the reference, which is forward-only): for the
+ * upstream gradient G = dL/dX ([batch][2n][m], strides sG_b, sG_r) writes dT = dL/dT. `ws` must be
+ * the workspace of the ssn_solve call that produced X, unmodified since. Implicit differentiation
+ * through the dual optimality condition: cross Gram M = B^H G_c (streaming), per target
+ * g = skew(-Im(N M N)), lam = H^-1 g by PCG (the forward's Hessian-vector product and
+ * preconditioner, relative residual 1e-13), K = N lam N, then dL/dB = G_c N + 2i B K (streaming).
+ * counters [batch][4] (may be NULL): adjoint CG iterations, 0 if the Hessian solve converged,
+ * device ticks (10 ns), 0. */
+int ssn_backward(void* stream, const double* T, long long sT_b, long long sT_r,
+                 const double* G, long long sG_b, long long sG_r, int batch, int n, int m,
+                 double* dT, long long sD_b, long long sD_r, int* counters, void* ws, size_t ws_bytes);
+
 #ifdef __cplusplus
 }
 #endif

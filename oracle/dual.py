@@ -195,12 +195,14 @@ class DualStats:
     failed: np.ndarray = field(default=None)
 
 
-def dual_batch(Cr, Ci, raise_on_failure=True):
+def dual_batch(Cr, Ci, raise_on_failure=True, feas_tol=None):
     """Newton / PCG / Armijo on the dual (dual_recommended_hvp.py:99-191).
 
     Returns (omega, L, iterations, P, Q, stats); P, Q are the final inverse parts in
-    the original basis (the reference's `final` cache, :131-133).
+    the original basis (the reference's `final` cache, :131-133). `feas_tol` overrides
+    FEAS_TOL (the tests tighten it to converge the dual to rounding level).
     """
+    feas_tol = FEAS_TOL if feas_tol is None else feas_tol
     batch, m, _ = Cr.shape
     I = np.broadcast_to(np.eye(m), (batch, m, m))
     omega = np.zeros_like(Cr)
@@ -233,7 +235,7 @@ def dual_batch(Cr, Ci, raise_on_failure=True):
         gradient = Jscaled - Si                                                 # :126
         gradient_norm = norm(gradient)
         residual = gradient_norm * scale / m ** 0.5                             # :128
-        active &= residual >= FEAS_TOL / 2                                      # :129
+        active &= residual >= feas_tol / 2                                      # :129
         if not bool(active.any()):                                              # :130
             omega, L, P, Q = from_eigenbasis(omega, P, Q, vectors)
             stats.newton[:] = iterations
@@ -310,13 +312,13 @@ def recover(T, P, Q):
     return np.concatenate((top, bot), axis=-2)
 
 
-def solve(T, return_stats=False):
+def solve(T, return_stats=False, feas_tol=None):
     """(X, omega, iterations) for targets T [batch, 2n, 2k] (dual_recommended_hvp.py:231-242)."""
     T = np.asarray(T, dtype=np.float64)
     if T.ndim != 3 or T.shape[-2] % 2:
         raise ValueError("T must be [batch, 2n, 2k]")
     Cr, Ci = gram(T)
-    omega, L, iterations, P, Q, stats = dual_batch(Cr, Ci)
+    omega, L, iterations, P, Q, stats = dual_batch(Cr, Ci, feas_tol=feas_tol)
     X = recover(T, P, Q)
     if return_stats:
         return X, omega, iterations, stats

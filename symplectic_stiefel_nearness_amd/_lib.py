@@ -38,6 +38,7 @@ SIGNATURES = {
     "ssn_dual_newton": (_I, [_P, _P, _P, _L, _I, _I, _I, _P, _L, _I, _P, _P, _P, _P, _P, _P, _S]),
     "ssn_recover": (_I, [_P, _P, _L, _L, _P, _P, _I, _I, _I, _P, _L, _L, _P, _S]),
     "ssn_solve": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _L, _I, _P, _P, _P, _P, _S]),
+    "ssn_backward": (_I, [_P, _P, _L, _L, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _P, _S]),
 }
 
 _lib = None

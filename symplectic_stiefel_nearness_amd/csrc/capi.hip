@@ -17,6 +17,13 @@ size_t dual_workspace_doubles(int batch, int m);
 hipError_t launch_dual(hipStream_t, const double*, const double*, long long, int, int, int,
                        double*, long long, int, double*, double*, long long*, int*, int*, double*, size_t);
 hipError_t launch_dual_eigh(hipStream_t, const double*, long long, int, int, int, double*, size_t);
+hipError_t launch_cross_gram(hipStream_t, const double*, long long, long long, const double*, long long, long long,
+                             int, int, int, int, double*, double*, double*, long long);
+hipError_t launch_adjoint(hipStream_t, double*, const double*, const double*, int, int, int*, double, size_t);
+hipError_t launch_backward_stream(hipStream_t, const double*, long long, long long, const double*, long long,
+                                  long long, const double*, const double*, long long, const double*,
+                                  const double*, double*, int, int, int, double*, long long, long long);
+long long workspace_target_doubles(int m);
 hipError_t launch_dual_newton(hipStream_t, const double*, const double*, long long, int, int, int,
                               double*, long long, int, double*, double*, long long*, int*, int*, double*, size_t);
 }  // namespace ssn
@@ -29,6 +36,12 @@ using namespace ssn;
 #define SSN_ERR_WORKSPACE (-3)
 #define SSN_ERR_UNSUPPORTED (-4)
 #define SSN_ERR_INIT (-5)
+
+// solver workspace slots used by the backward (dual.hip: S_G, S_D, S_R, S_Z0)
+#define SLOT_MR 8
+#define SLOT_MI 9
+#define SLOT_KR 10
+#define SLOT_KI 11
 
 namespace {
 
@@ -61,7 +74,8 @@ bool shape_ok(int batch, int n, int m) {
     return batch >= 0 && n >= 1 && m >= 2 && (m % 2) == 0 && m <= 128;
 }
 
-// workspace layout (doubles): [gram partials][Cr][Ci][dual slots][Pf][Qf][recovery fragments]
+// workspace layout (doubles): [gram partials][Cr][Ci][dual slots][Pf][Qf][fragments]; the
+// fragment region holds the recovery's 2mp x 2mp or the backward's 4mp x 2mp matrices
 struct Layout {
     size_t partial, cr, ci, dual, pf, qf, frag, total;
     int chunks;
@@ -78,7 +92,7 @@ Layout layout(int batch, int n, int m) {
     L.dual = off;    off += dual_workspace_doubles(batch, m);
     L.pf = off;      off += (size_t)batch * mp * mp;
     L.qf = off;      off += (size_t)batch * mp * mp;
-    L.frag = off;    off += (size_t)batch * 4 * mp * mp;
+    L.frag = off;    off += (size_t)batch * 8 * mp * mp;
     L.total = off;
     return L;
 }
@@ -210,6 +224,33 @@ int ssn_solve(void* stream, const double* T, long long sT_b, long long sT_r, int
     rc = hip_status(launch_recovery_fragments(s, w + L.pf, w + L.qf, batch, m, w + L.frag));
     if (rc) return rc;
     return hip_status(launch_recover(s, T, sT_b, sT_r, w + L.frag, batch, n, m, X, sX_b, sX_r));
+}
+
+int ssn_backward(void* stream, const double* T, long long sT_b, long long sT_r,
+                 const double* G, long long sG_b, long long sG_r, int batch, int n, int m,
+                 double* dT, long long sD_b, long long sD_r, int* counters, void* ws, size_t ws_bytes)
+{
+    if (!shape_ok(batch, n, m) || !T || !G || !dT) return SSN_ERR_ARGS;
+    if (batch == 0) return SSN_OK;
+    size_t lds = 0;
+    int rc = current_device_ready(&lds);
+    if (rc) return rc;
+    Layout L = layout(batch, n, m);
+    if (!ws || ws_bytes < L.total * sizeof(double)) return SSN_ERR_WORKSPACE;
+    hipStream_t s = (hipStream_t)stream;
+    double* w = (double*)ws;
+    double* dual = w + L.dual;
+    const long long td = workspace_target_doubles(m);
+    const long long mm = (long long)pad16(m) * pad16(m);
+    // cross Gram M = B^H G_c into the solver slots of M's real / imaginary parts
+    rc = hip_status(launch_cross_gram(s, T, sT_b, sT_r, G, sG_b, sG_r, batch, n, m, L.chunks, w + L.partial,
+                                      dual + SLOT_MR * mm, dual + SLOT_MI * mm, td));
+    if (rc) return rc;
+    rc = hip_status(launch_adjoint(s, dual, w + L.pf, w + L.qf, batch, m, counters, 1e-13, lds));
+    if (rc) return rc;
+    return hip_status(launch_backward_stream(s, T, sT_b, sT_r, G, sG_b, sG_r, dual + SLOT_KR * mm,
+                                             dual + SLOT_KI * mm, td, w + L.pf, w + L.qf, w + L.frag,
+                                             batch, n, m, dT, sD_b, sD_r));
 }
 
 }  // extern "C"

@@ -177,6 +177,137 @@ __global__ void gram_reduce_kernel(const double* __restrict__ partial, int chunk
     Ci[(long long)b * sC_b + (long long)i * ldc + j] = (i == j) ? 0.0 : (i < j ? ci_upper : -ci_upper);
 }
 
+// ---------------------------------------------------------------- cross Gram (backward)
+// M = B^H G_c for B = A + iD and G_c = G_top + i G_bot:
+//   Mr = A^T G_top + D^T G_bot,   Mi = A^T G_bot - D^T G_top   (full mp x mp).
+// Same streaming structure as the Gram: a 16-row LDS stage of A, D, G_top, G_bot per
+// step, 8 waves each owning a fixed list of output tiles (both parts accumulated).
+template <int NT>
+struct CrossShape {
+    static constexpr int MP = NT * 16;
+    static constexpr int NTOT = NT * NT;
+    static constexpr int MAXT = (NTOT + GRAM_WAVES - 1) / GRAM_WAVES;
+    static constexpr int LDS_STRIDE = MP + 2;
+    static constexpr int STAGE = 4 * GRAM_ROWS * LDS_STRIDE;   // A, D, Gt, Gb rows
+    static constexpr int PER_THREAD = (4 * GRAM_ROWS * MP + GRAM_THREADS - 1) / GRAM_THREADS;
+};
+
+template <int NT>
+__global__ __launch_bounds__(GRAM_THREADS)
+void cross_gram_partial_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
+                               const double* __restrict__ G, long long sG_b, long long sG_r,
+                               int n, int m, int chunk_rows, int chunks, double* __restrict__ partial)
+{
+    using S = CrossShape<NT>;
+    constexpr int MP = S::MP;
+    __shared__ double stage[2][S::STAGE];
+    const int chunk = blockIdx.x, b = blockIdx.y;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int r = lane & 15, q = lane >> 4;
+    const int row_begin = chunk * chunk_rows;
+    const int row_end = min(n, row_begin + chunk_rows);
+    const double* Tb = T + (long long)b * sT_b;
+    const double* Gb = G + (long long)b * sG_b;
+
+    d4 accr[S::MAXT], acci[S::MAXT];
+#pragma unroll
+    for (int s = 0; s < S::MAXT; ++s) { accr[s] = zero4(); acci[s] = zero4(); }
+
+    double pre[S::PER_THREAD];
+    auto load_stage = [&](int r0) {
+#pragma unroll
+        for (int s = 0; s < S::PER_THREAD; ++s) {
+            const int e = tid + s * GRAM_THREADS;
+            double v = 0.0;
+            if (e < 4 * GRAM_ROWS * MP) {
+                const int part = e / (GRAM_ROWS * MP);          // 0 A, 1 D, 2 Gt, 3 Gb
+                const int rem = e - part * GRAM_ROWS * MP;
+                const int rr = rem / MP, cc = rem - (rem / MP) * MP;
+                const int row = r0 + rr;
+                if (row < row_end && cc < m) {
+                    const int grow = row + (part & 1) * n;
+                    v = (part < 2) ? Tb[(long long)grow * sT_r + cc] : Gb[(long long)grow * sG_r + cc];
+                }
+            }
+            pre[s] = v;
+        }
+    };
+    auto store_stage = [&](double* dst) {
+#pragma unroll
+        for (int s = 0; s < S::PER_THREAD; ++s) {
+            const int e = tid + s * GRAM_THREADS;
+            if (e < 4 * GRAM_ROWS * MP) {
+                const int part = e / (GRAM_ROWS * MP);
+                const int rem = e - part * GRAM_ROWS * MP;
+                const int rr = rem / MP, cc = rem - (rem / MP) * MP;
+                dst[(part * GRAM_ROWS + rr) * S::LDS_STRIDE + cc] = pre[s];
+            }
+        }
+    };
+    const int nstages = (row_end - row_begin + GRAM_ROWS - 1) / GRAM_ROWS;
+    if (nstages > 0) load_stage(row_begin);
+    for (int st = 0; st < nstages; ++st) {
+        double* cur = stage[st & 1];
+        store_stage(cur);
+        __syncthreads();
+        if (st + 1 < nstages) load_stage(row_begin + (st + 1) * GRAM_ROWS);
+        const double* sA = cur;
+        const double* sD = cur + GRAM_ROWS * S::LDS_STRIDE;
+        const double* sGt = cur + 2 * GRAM_ROWS * S::LDS_STRIDE;
+        const double* sGb = cur + 3 * GRAM_ROWS * S::LDS_STRIDE;
+#pragma unroll 1
+        for (int kq = 0; kq < GRAM_ROWS / 4; ++kq) {
+            int row = (kq * 4 + q) * S::LDS_STRIDE + r;
+            asm volatile("" : "+v"(row));
+#pragma unroll
+            for (int s = 0; s < S::MAXT; ++s) {
+                const int t = wave + s * GRAM_WAVES;
+                if (t < S::NTOT) {
+                    const int I = (t / NT) * 16, J = (t % NT) * 16;
+                    const double aI = sA[row + I], dI = sD[row + I];
+                    const double gt = sGt[row + J], gb = sGb[row + J];
+                    accr[s] = mfma(aI, gt, accr[s]);
+                    accr[s] = mfma(dI, gb, accr[s]);
+                    acci[s] = mfma(aI, gb, acci[s]);
+                    acci[s] = mfma(-dI, gt, acci[s]);
+                }
+            }
+        }
+    }
+    double* out = partial + ((long long)b * chunks + chunk) * 2 * MP * MP;
+#pragma unroll
+    for (int s = 0; s < S::MAXT; ++s) {
+        const int t = wave + s * GRAM_WAVES;
+        if (t < S::NTOT) {
+            const int I = t / NT, J = t % NT;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int o = (I * 16 + q + 4 * i) * MP + J * 16 + r;
+                out[o] = accr[s][i];
+                out[MP * MP + o] = acci[s][i];
+            }
+        }
+    }
+}
+
+// Sums the chunk partials in ascending order into Mr, Mi (each [mp][mp] at the given
+// per-target strides, typically inside the solver workspace).
+__global__ void cross_gram_reduce_kernel(const double* __restrict__ partial, int chunks, int mp,
+                                         double* __restrict__ Mr, double* __restrict__ Mi, long long sM_b)
+{
+    const int b = blockIdx.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mp * mp) return;
+    const double* p = partial + (long long)b * chunks * 2 * mp * mp;
+    double re = 0.0, im = 0.0;
+    for (int c = 0; c < chunks; ++c) {
+        re += p[(long long)c * 2 * mp * mp + e];
+        im += p[(long long)c * 2 * mp * mp + mp * mp + e];
+    }
+    Mr[(long long)b * sM_b + e] = re;
+    Mi[(long long)b * sM_b + e] = im;
+}
+
 }  // namespace ssn
 
 // ------------------------------------------------------------------ launchers
@@ -209,6 +340,31 @@ hipError_t launch_gram_partial(hipStream_t stream, const double* T, long long sT
         default: return hipErrorInvalidValue;
     }
 #undef SSN_GRAM_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_cross_gram(hipStream_t stream, const double* T, long long sT_b, long long sT_r,
+                             const double* G, long long sG_b, long long sG_r, int batch, int n, int m,
+                             int chunks, double* partial, double* Mr, double* Mi, long long sM_b)
+{
+    const int mp = pad16(m);
+    const int nt = mp / 16;
+    int chunk_rows = (n + chunks - 1) / chunks;
+    chunk_rows = (chunk_rows + GRAM_ROWS - 1) / GRAM_ROWS * GRAM_ROWS;
+    dim3 grid(chunks, batch), block(GRAM_THREADS);
+#define SSN_CROSS_CASE(NTV) \
+    case NTV: hipLaunchKernelGGL(cross_gram_partial_kernel<NTV>, grid, block, 0, stream, \
+                                 T, sT_b, sT_r, G, sG_b, sG_r, n, m, chunk_rows, chunks, partial); break;
+    switch (nt) {
+        SSN_CROSS_CASE(1) SSN_CROSS_CASE(2) SSN_CROSS_CASE(3) SSN_CROSS_CASE(4)
+        SSN_CROSS_CASE(5) SSN_CROSS_CASE(6) SSN_CROSS_CASE(7) SSN_CROSS_CASE(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef SSN_CROSS_CASE
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 g2((mp * mp + 255) / 256, batch);
+    hipLaunchKernelGGL(cross_gram_reduce_kernel, g2, dim3(256), 0, stream, partial, chunks, mp, Mr, Mi, sM_b);
     return hipGetLastError();
 }
 

@@ -38,15 +38,20 @@ __global__ void build_recovery_fragments(const double* __restrict__ P, const dou
     frag[(long long)b * m2 * m2 + e] = v;
 }
 
-template <int NT>
+// Streaming product of the rows of up to two 2n x m inputs with a fragment-ordered matrix:
+//   [out_top | out_bot](r) = [a_r | d_r (| gt_r | gb_r)] * M,   M of size (2 NIN mp) x 2mp,
+// where a/d (gt/gb) are the top/bottom halves of T (G). NIN = 1 is the recovery, NIN = 2 the
+// backward product dL/dT = [Re; Im](G_c N + 2i B K).
+template <int NT, int NIN>
 __global__ __launch_bounds__(REC_THREADS)
-void recover_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
-                    const double* __restrict__ frag, int n, int m,
-                    double* __restrict__ X, long long sX_b, long long sX_r)
+void stream_gemm_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
+                        const double* __restrict__ G, long long sG_b, long long sG_r,
+                        const double* __restrict__ frag, int n, int m,
+                        double* __restrict__ X, long long sX_b, long long sX_r)
 {
     constexpr int MP = NT * 16;
-    constexpr int NJ = 2 * NT;          // output tiles per row block (top + bottom)
-    constexpr int KS = 2 * MP / 4;      // k-steps
+    constexpr int NJ = 2 * NT;            // output tiles per row block (top + bottom)
+    constexpr int KS = 2 * NIN * MP / 4;  // k-steps
     const int b = blockIdx.y;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = lane & 15, q = lane >> 4;
@@ -55,7 +60,14 @@ void recover_kernel(const double* __restrict__ T, long long sT_b, long long sT_r
     const double* Tb = T + (long long)b * sT_b;
     const double* Arow = Tb + (long long)row * sT_r;
     const double* Drow = Tb + (long long)(row + n) * sT_r;
-    const double* F = frag + (long long)b * (2 * MP) * (2 * MP);
+    const double* Gtrow = nullptr;
+    const double* Gbrow = nullptr;
+    if (NIN == 2) {
+        const double* Gb = G + (long long)b * sG_b;
+        Gtrow = Gb + (long long)row * sG_r;
+        Gbrow = Gb + (long long)(row + n) * sG_r;
+    }
+    const double* F = frag + (long long)b * (2 * NIN * MP) * (2 * MP);
 
     d4 acc[NJ];
 #pragma unroll
@@ -64,10 +76,11 @@ void recover_kernel(const double* __restrict__ T, long long sT_b, long long sT_r
 #pragma unroll 2
     for (int ks = 0; ks < KS; ++ks) {
         const int k = 4 * ks + q;
+        const int part = k / MP, kk = k - part * MP;     // 0 a, 1 d, 2 gt, 3 gb
         double a = 0.0;
-        if (row_ok) {
-            if (k < MP) { if (k < m) a = Arow[k]; }
-            else if (k - MP < m) a = Drow[k - MP];
+        if (row_ok && kk < m) {
+            const double* src = part == 0 ? Arow : part == 1 ? Drow : part == 2 ? Gtrow : Gbrow;
+            a = src[kk];
         }
         const double* Fk = F + (long long)ks * NJ * 64 + lane;
 #pragma unroll
@@ -90,6 +103,40 @@ void recover_kernel(const double* __restrict__ T, long long sT_b, long long sT_r
     }
 }
 
+// Backward fragments: M_bwd (4mp x 2mp) with row blocks
+//   a: [-2 Ki, 2 Kr],  d: [-2 Kr, -2 Ki],  gt: [P, Q],  gb: [-Q, P]
+// so that [a | d | gt | gb] M_bwd = [Re | Im](G_c N + 2i B K), in MFMA B-fragment order.
+// Kr, Ki are read at batch stride sK_b; P, Q are [batch][mp][mp].
+__global__ void build_backward_fragments(const double* __restrict__ Kr, const double* __restrict__ Ki,
+                                         long long sK_b, const double* __restrict__ P,
+                                         const double* __restrict__ Q, int mp, double* __restrict__ frag)
+{
+    const int b = blockIdx.y;
+    const int rows = 4 * mp, cols = 2 * mp;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * cols) return;
+    const int lane = e & 63;
+    const int J = (e >> 6) % (cols / 16);
+    const int ks = (e >> 6) / (cols / 16);
+    const int k = 4 * ks + (lane >> 4), j = 16 * J + (lane & 15);
+    const int part = k / mp, kk = k - part * mp;
+    const bool left = j < mp;
+    const int jj = left ? j : j - mp;
+    const double* kr = Kr + (long long)b * sK_b;
+    const double* ki = Ki + (long long)b * sK_b;
+    const double* Pb = P + (long long)b * mp * mp;
+    const double* Qb = Q + (long long)b * mp * mp;
+    const int o = kk * mp + jj;
+    double v;
+    switch (part) {
+        case 0: v = left ? -2.0 * ki[o] : 2.0 * kr[o]; break;
+        case 1: v = left ? -2.0 * kr[o] : -2.0 * ki[o]; break;
+        case 2: v = left ? Pb[o] : Qb[o]; break;
+        default: v = left ? -Qb[o] : Pb[o]; break;
+    }
+    frag[(long long)b * rows * cols + e] = v;
+}
+
 hipError_t launch_recovery_fragments(hipStream_t stream, const double* P, const double* Q,
                                      int batch, int m, double* frag)
 {
@@ -100,15 +147,16 @@ hipError_t launch_recovery_fragments(hipStream_t stream, const double* P, const 
     return hipGetLastError();
 }
 
-hipError_t launch_recover(hipStream_t stream, const double* T, long long sT_b, long long sT_r,
-                          const double* frag, int batch, int n, int m,
-                          double* X, long long sX_b, long long sX_r)
+template <int NIN>
+static hipError_t launch_stream(hipStream_t stream, const double* T, long long sT_b, long long sT_r,
+                                const double* G, long long sG_b, long long sG_r, const double* frag,
+                                int batch, int n, int m, double* X, long long sX_b, long long sX_r)
 {
     const int nt = pad16(m) / 16;
     dim3 grid((n + REC_ROWS - 1) / REC_ROWS, batch), block(REC_THREADS);
 #define SSN_REC_CASE(NTV) \
-    case NTV: hipLaunchKernelGGL(recover_kernel<NTV>, grid, block, 0, stream, \
-                                 T, sT_b, sT_r, frag, n, m, X, sX_b, sX_r); break;
+    case NTV: hipLaunchKernelGGL((stream_gemm_kernel<NTV, NIN>), grid, block, 0, stream, \
+                                 T, sT_b, sT_r, G, sG_b, sG_r, frag, n, m, X, sX_b, sX_r); break;
     switch (nt) {
         SSN_REC_CASE(1) SSN_REC_CASE(2) SSN_REC_CASE(3) SSN_REC_CASE(4)
         SSN_REC_CASE(5) SSN_REC_CASE(6) SSN_REC_CASE(7) SSN_REC_CASE(8)
@@ -116,6 +164,28 @@ hipError_t launch_recover(hipStream_t stream, const double* T, long long sT_b, l
     }
 #undef SSN_REC_CASE
     return hipGetLastError();
+}
+
+hipError_t launch_recover(hipStream_t stream, const double* T, long long sT_b, long long sT_r,
+                          const double* frag, int batch, int n, int m,
+                          double* X, long long sX_b, long long sX_r)
+{
+    return launch_stream<1>(stream, T, sT_b, sT_r, nullptr, 0, 0, frag, batch, n, m, X, sX_b, sX_r);
+}
+
+hipError_t launch_backward_stream(hipStream_t stream, const double* T, long long sT_b, long long sT_r,
+                                  const double* G, long long sG_b, long long sG_r,
+                                  const double* Kr, const double* Ki, long long sK_b,
+                                  const double* P, const double* Q, double* frag,
+                                  int batch, int n, int m, double* dT, long long sD_b, long long sD_r)
+{
+    const int mp = pad16(m);
+    const int total = 8 * mp * mp;
+    hipLaunchKernelGGL(build_backward_fragments, dim3((total + 255) / 256, batch), dim3(256), 0, stream,
+                       Kr, Ki, sK_b, P, Q, mp, frag);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_stream<2>(stream, T, sT_b, sT_r, G, sG_b, sG_r, frag, batch, n, m, dT, sD_b, sD_r);
 }
 
 }  // namespace ssn

@@ -78,6 +78,85 @@ def _workspace(nbytes, device):
     return torch.empty(max(int(nbytes), 8), dtype=torch.uint8, device=device)
 
 
+def _solve_device_ws(T, check=True):
+    """Device solve that also returns the workspace holding the forward state (for backward)."""
+    lib = _require_ready()
+    batch, n, m = _check_targets(T)
+    dev = T.device
+    X = torch.empty((batch, 2 * n, m), dtype=torch.float64, device=dev)
+    omega = torch.empty((batch, m, m), dtype=torch.float64, device=dev)
+    iters = torch.zeros(batch, dtype=torch.int64, device=dev)
+    status = torch.zeros(batch, dtype=torch.int32, device=dev)
+    counters = torch.zeros((batch, 12), dtype=torch.int32, device=dev)
+    ws = _workspace(lib.ssn_workspace_bytes(batch, n, m) if batch else 8, dev)
+    if batch:
+        with torch.cuda.device(dev):
+            rc = lib.ssn_solve(_stream(), T.data_ptr(), T.stride(0), T.stride(1), batch, n, m,
+                               X.data_ptr(), X.stride(0), X.stride(1),
+                               omega.data_ptr(), omega.stride(0), omega.stride(1),
+                               iters.data_ptr(), status.data_ptr(), counters.data_ptr(),
+                               ws.data_ptr(), ws.numel())
+        _lib.check(rc, "ssn_solve")
+    if check:
+        _raise_for_status(status)
+    return X, omega, iters, ws, dict(status=status, counters=counters)
+
+
+def backward_device(T, G, ws, return_stats=False):
+    """dL/dT for the upstream gradient G = dL/dX, from the forward workspace `ws` of T's solve."""
+    lib = _require_ready()
+    batch, n, m = _check_targets(T)
+    assert G.shape == T.shape and G.dtype == torch.float64 and G.device == T.device
+    G = G.contiguous()
+    dT = torch.empty_like(T)
+    counters = torch.zeros((batch, 4), dtype=torch.int32, device=T.device)
+    if batch:
+        with torch.cuda.device(T.device):
+            rc = lib.ssn_backward(_stream(), T.data_ptr(), T.stride(0), T.stride(1),
+                                  G.data_ptr(), G.stride(0), G.stride(1), batch, n, m,
+                                  dT.data_ptr(), dT.stride(0), dT.stride(1), counters.data_ptr(),
+                                  ws.data_ptr(), ws.numel())
+        _lib.check(rc, "ssn_backward")
+    if return_stats:
+        return dT, dict(counters=counters)
+    return dT
+
+
+class _Projection(torch.autograd.Function):
+    """X = P(T) with a backward by implicit differentiation of the dual optimality condition."""
+
+    @staticmethod
+    def forward(ctx, T):
+        T = T.contiguous()
+        X, omega, iters, ws, _ = _solve_device_ws(T)
+        ctx.save_for_backward(T)
+        ctx.ws = ws
+        ctx.mark_non_differentiable(omega, iters)
+        return X, omega, iters
+
+    @staticmethod
+    def backward(ctx, gX, gomega, giters):
+        (T,) = ctx.saved_tensors
+        if gX is None:
+            return None
+        return backward_device(T, gX, ctx.ws)
+
+
+def project(T):
+    """Differentiable projection: (X, omega, iterations) = solve(T) with dX/dT available to autograd.
+
+    Gradients flow from X only (omega and the iteration counts are non-differentiable outputs).
+    Device input runs in place; CPU input is moved to the current device and the outputs back.
+    """
+    _require_ready()
+    _check_targets(T)
+    if T.is_cuda:
+        return _Projection.apply(T)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    X, omega, iters = _Projection.apply(T.to(dev))
+    return X.cpu(), omega.cpu(), iters.cpu()
+
+
 @torch.no_grad()
 def solve_device(T, return_stats=False, check=True):
     """Device-resident solve: T is a float64 HIP tensor [batch, 2n, 2k]; outputs stay on the device.

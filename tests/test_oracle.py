@@ -283,3 +283,57 @@ def test_batched_equals_per_target():
         assert np.array_equal(ib[j:j + 1], is_)
         assert ss.cg[0] == sb.cg[j] and ss.backtracks[0] == sb.backtracks[j]
         assert np.linalg.norm(Xs[0] - Xb[j]) <= 1e-13 * np.linalg.norm(Xb[j])
+
+
+# ------------------------------------------------------------------ adjoint (backward)
+
+from oracle import adjoint as oadj
+
+
+@pytest.mark.parametrize("n,k,seed", [(6, 1, 0), (20, 3, 1), (40, 4, 2)])
+def test_adjoint_matches_central_differences(n, k, seed):
+    """<dL/dT, E> against (L(T + hE) - L(T - hE)) / 2h, dual converged to rounding level."""
+    g = rng(200 + seed)
+    T = g.standard_normal((1, 2 * n, 2 * k)) / math.sqrt(2 * n)
+    G = g.standard_normal((1, 2 * n, 2 * k))
+    dT, X, omega = oadj.projection_grad_batch(T, G)
+    h = 1e-6
+    for _ in range(4):
+        E = g.standard_normal(T.shape)
+        Xp, _, _ = odual.solve(T + h * E, feas_tol=1e-12)
+        Xm, _, _ = odual.solve(T - h * E, feas_tol=1e-12)
+        fd = (np.sum(G * Xp) - np.sum(G * Xm)) / (2 * h)
+        an = np.sum(dT * E)
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (fd, an)
+
+
+def test_adjoint_orthogonal_to_the_manifold_point_at_T_eq_cS():
+    """X(cS) = S for every c in (0, 2), so <dL/dT, S> = 0 at T = cS for every G."""
+    g = rng(11)
+    n, k = 16, 2
+    S = orthosymplectic(g, n, k)
+    for c in (0.6, 1.4):
+        G = g.standard_normal((1, 2 * n, 2 * k))
+        dT, _, _ = oadj.projection_grad_batch((c * S)[None], G)
+        assert abs(np.sum(dT[0] * S)) < 1e-9 * np.linalg.norm(dT)
+
+
+def test_adjoint_hessian_solve_is_exact():
+    """The PCG inside the adjoint solves H lam = g to 1e-14 (checked with a dense H)."""
+    g = rng(12)
+    n, k = 10, 2
+    T = g.standard_normal((1, 2 * n, 2 * k)) / math.sqrt(2 * n)
+    _, omega, _ = odual.solve(T, feas_tol=1e-12)
+    m = 2 * k
+    B = T[0, :n] + 1j * T[0, n:]
+    N = np.linalg.inv(np.eye(m) + 1j * omega[0])
+    C = B.conj().T @ B
+    S = N @ C @ N
+    rhs = _skew_np(g.standard_normal((m, m)))
+    lam = oadj.solve_hessian(N.real, N.imag, S.real, S.imag, C.real, rhs)
+    res = oadj.hvp_unscaled(N.real, N.imag, S.real, S.imag, lam) - rhs
+    assert np.linalg.norm(res) < 1e-12 * np.linalg.norm(rhs)
+
+
+def _skew_np(a):
+    return (a - a.T) / 2
