@@ -147,6 +147,44 @@ def main():
         plain_ms = sum(runs) / len(runs)
         log(f"plain per-target loop: {plain_ms:.2f} ms for {BATCH} targets")
 
+    # forward + backward through the autograd projection, upstream gradient G ~ N(0, 1)
+    G = inputs.make_gradients(N, K, BATCH).to(dev)
+    Tg = T.clone().requires_grad_(True)
+
+    def fwd_bwd():
+        Tg.grad = None
+        X_, _, _ = ssn.project(Tg)
+        torch.autograd.backward(X_, G)
+        return Tg.grad
+
+    for _ in range(2):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    fb_reps = 5
+    e0 = [torch.cuda.Event(enable_timing=True) for _ in range(fb_reps + 1)]
+    t4 = time.perf_counter()
+    for i in range(fb_reps):
+        e0[i].record(stream)
+        fwd_bwd()
+    e0[fb_reps].record(stream)
+    torch.cuda.synchronize()
+    fb_ms = (time.perf_counter() - t4) * 1e3 / fb_reps
+    # backward alone (same forward state), HIP events on the launch stream
+    X_, om_, it_, ws_, _ = ssn.dual._solve_device_ws(T)
+    eb = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ssn.backward_device(T, G, ws_)
+    torch.cuda.synchronize()
+    eb[0].record(stream)
+    for _ in range(fb_reps):
+        dT, bst = ssn.backward_device(T, G, ws_, return_stats=True)
+    eb[1].record(stream)
+    torch.cuda.synchronize()
+    bwd_ms = eb[0].elapsed_time(eb[1]) / fb_reps
+    adj_it = bst["counters"][:, 0].double().mean().item()
+    adj_ok = bool((bst["counters"][:, 1] == 0).all().item())
+    del X_, om_, it_, ws_, dT, Tg
+    log(f"fwd+bwd: {fb_ms:.3f} ms (backward alone {bwd_ms:.3f} ms, adjoint CG {adj_it:.1f} it)")
+
     # reference protocol boundary: CPU tensor in -> CPU tensors out (PCIe included)
     ssn.solve(T_cpu)
     torch.cuda.synchronize()
@@ -172,7 +210,7 @@ def main():
     kernels = [roof("ssn::gram_partial_kernel (+ gram_reduce_kernel)", f_gram, gram_ms),
                roof("ssn::eigh_kernel", f_eigh, eigh_ms),
                roof("ssn::dual_kernel", fd, newton_ms),
-               roof("ssn::recover_kernel (+ build_recovery_fragments)", f_rec, rec_ms)]
+               roof("ssn::stream_gemm_kernel<6,1> (+ build_recovery_fragments)", f_rec, rec_ms)]
     dominant = max(kernels, key=lambda k: k["avg_ms"])
     stages = {"gram": gram_ms, "eigh": eigh_ms, "newton": newton_ms, "recover": rec_ms}
 
@@ -194,7 +232,10 @@ def main():
         "plain_solves_per_s": BATCH / (plain_ms * 1e-3) if plain_ms else None,
         "single_call_solves_per_s": BATCH / (fused_ms * 1e-3),
         "pcie_inclusive_solves_per_s": BATCH / (pcie_ms * 1e-3),
-        "fwd_bwd_ms": None,
+        "fwd_bwd_ms": fb_ms,
+        "fwd_bwd_solves_per_s": BATCH / (fb_ms * 1e-3),
+        "bwd_ms": bwd_ms,
+        "adjoint": {"cg_mean": adj_it, "all_converged": adj_ok},
         "stage_ms": stages,
         "streaming_tflops": f_stream / ((gram_ms + rec_ms) * 1e-3) / 1e12,
         "dual_phase_ms": eigh_ms + newton_ms,
