@@ -1,0 +1,58 @@
+"""The C ABI (include/ssn.h) against libssn.so and its ctypes binding, without a GPU.
+
+Every function declared in the header is exported by the library and bound in
+symplectic_stiefel_nearness_amd/_lib.py with the same number of parameters; the
+host-only queries (version, padded order, status strings, workspace sizes) answer
+correctly. No kernel is launched here.
+"""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from symplectic_stiefel_nearness_amd import _lib
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "ssn.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(ssn_\w+)\s*\(([^)]*)\)\s*;", text):
+        params = m.group(2).strip()
+        out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libssn.so not built")
+    return ctypes.CDLL(str(_lib.LIB_PATH))
+
+
+def test_header_declares_the_bound_functions():
+    decl = declared_functions()
+    assert set(decl) == set(_lib.SIGNATURES), set(decl) ^ set(_lib.SIGNATURES)
+    for name, nparams in decl.items():
+        assert len(_lib.SIGNATURES[name][1]) == nparams, name
+
+
+def test_library_exports_every_declared_symbol(lib):
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_host_queries(lib):
+    L = _lib.load()
+    assert L.ssn_version().decode().startswith("ssn-gfx950")
+    assert [L.ssn_padded_order(m) for m in (2, 16, 17, 96, 128)] == [16, 16, 32, 96, 128]
+    assert L.ssn_status_string(_lib.ST_OK).decode() == "ok"
+    for code in (_lib.ST_NO_DESCENT, _lib.ST_NEWTON_CAP, _lib.ST_NONFINITE, _lib.ST_EIGH):
+        assert L.ssn_status_string(code).decode() not in ("", "ok")
+    assert L.ssn_workspace_bytes(64, 4096, 96) > 64 * 8 * 96 * 96
+    assert L.ssn_workspace_bytes(2, 100, 96) < L.ssn_workspace_bytes(4, 100, 96)
+    assert L.ssn_workspace_bytes(1, 10, 3) == 0          # odd order: unsupported
+    assert L.ssn_workspace_bytes(1, 10, 130) == 0        # above the largest order
