@@ -303,150 +303,281 @@ __device__ __forceinline__ void each(const Ctx& c, F f) {
     for (int e = opaque(c.tid); e < MP * MP; e += DT) f(e / MP, e % MP, e);
 }
 
+// 1 / sqrt(d): hardware estimate refined by two Newton steps (a few ulp).
+__device__ __forceinline__ double rsqrt_refined(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    y = y * (1.5 - 0.5 * d * y * y);
+    y = y * (1.5 - 0.5 * d * y * y);
+    return y;
+}
+
 // ---------------------------------------------------------------- Jacobi eigensolver
 // Cyclic two-sided Jacobi with the round-robin (circle) tournament: each round applies
-// m/2 disjoint rotations in parallel. A and V are kept in POSITION order: the pairs of
-// every round are the fixed slots (s, m-1-s), and the tournament's player rotation
-// (slot j -> j + 1 over 1..m-1, slot 0 fixed) is folded into the write-back of each
-// round. Every thread therefore owns fixed 2x2 blocks of A and fixed eigenvector
-// entries for the whole solve, and its LDS accesses run along rows (no bank conflicts).
+// m/2 disjoint rotations in parallel, on the fixed slot pairs (s, m-1-s); between rounds
+// the player in slot j moves to slot next(j) = j + 1 over 1..m-1 (slot 0 fixed, m-1 -> 1).
+//
+// A is kept in SLOT order, upper triangle only, as 2x2 pair blocks: block (P, P2), P <= P2,
+// holds rows (P, m-1-P) x columns (P2, m-1-P2), its top row pair in one double2 array and
+// its bottom row pair in another. Two buffers alternate: a round reads every block of one
+// and writes each rotated element straight to its place in the other after the slot move.
+// Since the move is the same every round, every read and write address is fixed for the
+// whole solve and computed once. Wave 0 owns the diagonal blocks; during round r it also
+// recomputes, from round r's inputs and with the same operations, the three elements each
+// pair of round r + 1 needs, and writes round r + 1's rotation parameters. One barrier per
+// round therefore suffices.
+//
+// V is kept in PLAYER order and updated in place (slot x holds player 0 if x = 0, else
+// 1 + ((x - 1 - r) mod (m - 1)) at round r of a sweep). After m - 1 rounds every player
+// is back in its own slot, so at sweep boundaries slot, player and column agree.
+//
 // A rotation is applied when |a_pq| > eps sqrt(|a_pp a_qq|); the iteration stops after
-// the first sweep at whose end no pair exceeds that threshold. On return the diagonal
-// of A holds the eigenvalues and the columns of V the eigenvectors, both in slot order.
-constexpr int JB_MAX = 5;     // 2x2 blocks per thread: ceil(64*65/2 / 512)
-constexpr int JV_MAX = 16;    // eigenvector pairs per thread: ceil(128*64 / 512)
+// the first sweep at whose end no pair exceeds that threshold.
+constexpr int JW = 64;        // wave 0: diagonal blocks and rotation parameters
+constexpr int JU = 512 - JW;  // threads that update off-diagonal blocks and V
+constexpr int JB_MAX = 5;     // off-diagonal blocks per update thread: ceil(64*63/2 / 448)
+constexpr int JB_BATCH = 3;   // blocks gathered before their stores
+constexpr int JV_HALF = 10;   // eigenvector rows per update thread: 2 x 10 >= ceil(128 / (448 / 64))
 constexpr double JAC_EPS = 2.220446049250313e-16;
 
-__device__ __forceinline__ int slot_next(int j, int m) { return j == 0 ? 0 : (j == m - 1 ? 1 : j + 1); }
+__host__ __device__ __forceinline__ int jac_blocks(int m) { return (m / 2) * (m / 2 + 1) / 2; }
+// LDS doubles of the Jacobi area: two A buffers (4 nb each), two parameter buffers
+// ([cos, sin] double2 x np, tan x np, pad np), then m ints of ranks
+__host__ __device__ __forceinline__ int jac_area(int m) { return 8 * jac_blocks(m) + 8 * (m / 2) + m / 2 + 2; }
 
-template <int TH>
-__device__ int jacobi(const Ctx& c, double* A, int LA, double* V, int LV, double* rot, int* flag) {
-    static_assert(TH == 512, "work lists are sized for 512 threads");
-    const int m = c.m, np = m / 2;
-    // rotation parameters as structure of arrays: consecutive lanes read consecutive slots
-    double* rc = rot;
-    double* rs = rot + np;
-    double* rt = rot + 2 * np;
-    for (int e = c.tid; e < m * m; e += TH) {
+__device__ __forceinline__ int player(int x, int r, int m) {
+    if (x == 0) return 0;
+    int y = x - 1 - r;
+    if (y < 0) y += m - 1;
+    return 1 + y;
+}
+
+// Rotation (cos, sin, tan) that annihilates a_pq: tan = sign(theta) / (|theta| + sqrt(theta^2 + 1)),
+// theta = (a_qq - a_pp) / (2 a_pq), evaluated without dividing by a_pq.
+__device__ __forceinline__ void jac_param(double app, double aqq, double apq, double& cs, double& sn, double& t) {
+    cs = 1.0; sn = 0.0; t = 0.0;
+    if (apq != 0.0 && fabs(apq) > JAC_EPS * sqrt(fabs(app) * fabs(aqq))) {
+        const double d = aqq - app, e = 2.0 * apq;
+        const double den = fabs(d) + sqrt(fma(d, d, e * e));
+        t = (den > 0.0) ? e / den : 1.0;
+        if (d < 0.0) t = -t;
+        cs = rsqrt_refined(fma(t, t, 1.0));
+        sn = t * cs;
+    }
+}
+
+// G1^T X G2 for the block X = [x00 x01; x10 x11] (rows rotated by r1, columns by r2).
+__device__ __forceinline__ void jac_block(double2 r1, double2 r2, double x00, double x01, double x10, double x11,
+                                          double& z00, double& z01, double& z10, double& z11) {
+    const double y00 = fma(r2.x, x00, -(r2.y * x01)), y01 = fma(r2.y, x00, r2.x * x01);
+    const double y10 = fma(r2.x, x10, -(r2.y * x11)), y11 = fma(r2.y, x10, r2.x * x11);
+    z00 = fma(r1.x, y00, -(r1.y * y10)); z10 = fma(r1.y, y00, r1.x * y10);
+    z01 = fma(r1.x, y01, -(r1.y * y11)); z11 = fma(r1.y, y01, r1.x * y11);
+}
+
+// Runs the sweeps on the slot-ordered A in J (buffer 0 filled by the caller) and the
+// player-ordered V (m x m, row stride LV, initialised here). Returns the sweep count
+// (-1 without convergence) and in `fin` the buffer holding the final A.
+template <int JB, bool V_IN_LDS>
+__device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int* flag, const double** fin) {
+    const int m = c.m, np = m / 2, nb = np * (np + 1) / 2;
+    double* A0 = J;
+    double* A1 = J + 4 * nb;
+    double* R0 = J + 8 * nb;
+    double* R1 = R0 + 4 * np;
+    auto pr = [&](int x) { return x < np ? x : m - 1 - x; };
+    auto nx = [&](int x) { return x == 0 ? 0 : (x == m - 1 ? 1 : x + 1); };
+    auto pv = [&](int x) { return x == 0 ? 0 : (x == 1 ? m - 1 : x - 1); };
+    auto blk = [&](int P, int P2) { return P * np - (P * (P - 1)) / 2 + (P2 - P); };
+    auto eoff = [&](int x, int y) {
+        int px = pr(x), hx = x >= np ? 1 : 0, py = pr(y), hy = y >= np ? 1 : 0;
+        if (px > py || (px == py && hx > hy)) { int u = px; px = py; py = u; u = hx; hx = hy; hy = u; }
+        return (hx ? 2 * nb : 0) + 2 * blk(px, py) + hy;
+    };
+    // value of slot element (x, y) after the round that reads Ao with parameters Ro
+    auto rotated = [&](const double* Ao, const double* Ro, int x, int y) -> double {
+        int px = pr(x), hx = x >= np ? 1 : 0, py = pr(y), hy = y >= np ? 1 : 0;
+        if (px > py || (px == py && hx > hy)) { int u = px; px = py; py = u; u = hx; hx = hy; hy = u; }
+        const double2* cs = reinterpret_cast<const double2*>(Ro);
+        const int b = blk(px, py);
+        const double2 tp = reinterpret_cast<const double2*>(Ao)[b];
+        if (px == py) {
+            if (hx != hy) return cs[px].y != 0.0 ? 0.0 : tp.y;
+            const double t = Ro[2 * np + px];
+            return hx ? fma(t, tp.y, Ao[2 * nb + 2 * b + 1]) : fma(-t, tp.y, tp.x);
+        }
+        const double2 bt = reinterpret_cast<const double2*>(Ao + 2 * nb)[b];
+        double z00, z01, z10, z11;
+        jac_block(cs[px], cs[py], tp.x, tp.y, bt.x, bt.y, z00, z01, z10, z11);
+        return hx ? (hy ? z11 : z10) : (hy ? z01 : z00);
+    };
+
+    for (int e = c.tid; e < m * m; e += 512) {
         const int i = e / m, j = e - (e / m) * m;
         V[i * LV + j] = (i == j) ? 1.0 : 0.0;
     }
-    // fixed work lists: upper pair-blocks (P <= P2) and eigenvector items (row i, pair s)
-    int bP[JB_MAX], bQ[JB_MAX];
-#pragma unroll
-    for (int k = 0; k < JB_MAX; ++k) {
-        const int e = c.tid + k * TH;
-        bP[k] = -1; bQ[k] = 0;
-        if (e < np * (np + 1) / 2) {
-            int P = 0, start = 0, len = np;
-            while (e >= start + len) { start += len; ++P; --len; }
-            bP[k] = P; bQ[k] = P + (e - start);
+    auto big = [&](double v, double di, double dj) {
+        return v != 0.0 && fabs(v) > JAC_EPS * sqrt(fabs(di) * fabs(dj));
+    };
+    // The two roles run separate copies of the sweep loop, so that each keeps only its own
+    // state in registers. Both pass the same barriers in the same order: one per round and
+    // one per sweep, after which the sweep's convergence flag (two slots, alternating) is
+    // read. Slot s & 1 is cleared by thread 0 at the start of sweep s and set during the
+    // check at its end; the previous reads of that slot lie at least one barrier behind.
+    int sweeps = -1, cur = 0;
+    if (c.wave == 0) {
+        // wave 0: diagonal block S and the slots whose elements pair S needs next round
+        const bool dg = c.tid < np;
+        const int S = dg ? c.tid : 0, Sq = m - 1 - S;
+        const int drd = 2 * blk(S, S);
+        const int dw0 = eoff(nx(S), nx(S)), dw1 = eoff(nx(S), nx(Sq)), dw2 = eoff(nx(Sq), nx(Sq));
+        const int lx = pv(S), ly = pv(Sq);
+        // A0 is ready (written by the caller before a barrier): parameters of the first round
+        if (dg) {
+            const double2 tp = reinterpret_cast<const double2*>(A0)[drd / 2];
+            double cs, sn, t;
+            jac_param(tp.x, A0[2 * nb + drd + 1], tp.y, cs, sn, t);
+            reinterpret_cast<double2*>(R0)[S] = make_double2(cs, sn);
+            R0[2 * np + S] = t;
         }
-    }
-    int vI[JV_MAX], vS[JV_MAX];
-#pragma unroll
-    for (int k = 0; k < JV_MAX; ++k) {
-        const int e = c.tid + k * TH;
-        vI[k] = -1; vS[k] = 0;
-        if (e < m * np) { vI[k] = e / np; vS[k] = e - (e / np) * np; }
-    }
-    sync();
-    for (int sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
+        sync();
+        for (int sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
+            if (c.tid == 0) flag[sweep & 1] = 0;
 #pragma unroll 1
-        for (int round = 0; round < m - 1; ++round) {
-            if (c.tid < np) {
-                const int p = c.tid, q = m - 1 - c.tid;
-                const double app = A[p * LA + p], aqq = A[q * LA + q], apq = A[p * LA + q];
-                double cs = 1.0, sn = 0.0, t = 0.0;
-                if (apq != 0.0 && fabs(apq) > JAC_EPS * sqrt(fabs(app) * fabs(aqq))) {
-                    const double theta = (aqq - app) / (2.0 * apq);
-                    t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
-                    if (theta < 0.0) t = -t;
-                    cs = 1.0 / sqrt(t * t + 1.0);
-                    sn = t * cs;
+            for (int round = 0; round < m - 1; ++round) {
+                const double* Ao = cur ? A1 : A0;
+                double* An = cur ? A0 : A1;
+                const double* Ro = cur ? R1 : R0;
+                double* Rn = cur ? R0 : R1;
+                if (dg) {
+                    const double2* cso = reinterpret_cast<const double2*>(Ro);
+                    const double2 tp = *reinterpret_cast<const double2*>(Ao + drd);
+                    const double d2 = Ao[2 * nb + drd + 1];
+                    const double t = Ro[2 * np + S], sn = cso[S].y;
+                    const double app = rotated(Ao, Ro, lx, lx), aqq = rotated(Ao, Ro, ly, ly);
+                    const double apq = rotated(Ao, Ro, lx, ly);
+                    double cn, sn2, tn;
+                    jac_param(app, aqq, apq, cn, sn2, tn);
+                    reinterpret_cast<double2*>(Rn)[S] = make_double2(cn, sn2);
+                    Rn[2 * np + S] = tn;
+                    An[dw0] = fma(-t, tp.y, tp.x);
+                    An[dw1] = sn != 0.0 ? 0.0 : tp.y;
+                    An[dw2] = fma(t, tp.y, d2);
                 }
-                rc[p] = cs; rs[p] = sn; rt[p] = t;
+                sync();
+                cur ^= 1;
             }
+            const double* Af = cur ? A1 : A0;
+            if (dg && big(Af[drd + 1], Af[drd], Af[2 * nb + drd + 1])) flag[sweep & 1] = 1;
             sync();
-            // read and rotate
-            double za[JB_MAX][4], zv[JV_MAX][2];
-#pragma unroll
-            for (int k = 0; k < JB_MAX; ++k) {
-                if (bP[k] < 0) continue;
-                // opaque: keeps the per-item addresses from being hoisted out of the
-                // round loop (they would all stay live and spill)
-                const int P = opaque(bP[k]), P2 = opaque(bQ[k]), q = m - 1 - P, q2 = m - 1 - P2;
-                const double cs = rc[P], sn = rs[P];
-                if (P == P2) {
-                    const double t = rt[P], apq = A[P * LA + q];
-                    za[k][0] = A[P * LA + P] - t * apq;
-                    za[k][3] = A[q * LA + q] + t * apq;
-                    za[k][1] = (sn != 0.0) ? 0.0 : apq;
-                    za[k][2] = za[k][1];
-                } else {
-                    const double c2 = rc[P2], s2 = rs[P2];
-                    const double x00 = A[P * LA + P2], x01 = A[P * LA + q2];
-                    const double x10 = A[q * LA + P2], x11 = A[q * LA + q2];
-                    // right rotation (columns P2, q2), then left rotation (rows P, q)
-                    const double y00 = c2 * x00 - s2 * x01, y01 = s2 * x00 + c2 * x01;
-                    const double y10 = c2 * x10 - s2 * x11, y11 = s2 * x10 + c2 * x11;
-                    za[k][0] = cs * y00 - sn * y10; za[k][2] = sn * y00 + cs * y10;
-                    za[k][1] = cs * y01 - sn * y11; za[k][3] = sn * y01 + cs * y11;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < JV_MAX; ++k) {
-                if (vI[k] < 0) continue;
-                const int s = opaque(vS[k]);
-                const double cs = rc[s], sn = rs[s];
-                const double* row = V + opaque(vI[k]) * LV;
-                const double vp = row[s], vq = row[m - 1 - s];
-                zv[k][0] = cs * vp - sn * vq;
-                zv[k][1] = sn * vp + cs * vq;
-            }
-            sync();
-            // write back into the next round's slots
-#pragma unroll
-            for (int k = 0; k < JB_MAX; ++k) {
-                if (bP[k] < 0) continue;
-                const int P = opaque(bP[k]), P2 = opaque(bQ[k]);
-                const int a0 = slot_next(P, m), a1 = slot_next(m - 1 - P, m);
-                const int b0 = slot_next(P2, m), b1 = slot_next(m - 1 - P2, m);
-                A[a0 * LA + b0] = za[k][0]; A[a0 * LA + b1] = za[k][1];
-                A[a1 * LA + b0] = za[k][2]; A[a1 * LA + b1] = za[k][3];
-                if (P != P2) {
-                    A[b0 * LA + a0] = za[k][0]; A[b1 * LA + a0] = za[k][1];
-                    A[b0 * LA + a1] = za[k][2]; A[b1 * LA + a1] = za[k][3];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < JV_MAX; ++k) {
-                if (vI[k] < 0) continue;
-                const int s = opaque(vS[k]);
-                double* row = V + opaque(vI[k]) * LV;
-                row[slot_next(s, m)] = zv[k][0];
-                row[slot_next(m - 1 - s, m)] = zv[k][1];
-            }
-            sync();
+            if (!flag[sweep & 1]) { sweeps = sweep + 1; break; }
         }
-        // convergence: does any pair still exceed the rotation threshold?
-        if (c.tid == 0) *flag = 0;
-        sync();
-        int any = 0;
-        for (int e = c.tid; e < m * m; e += TH) {
-            const int i = e / m, j = e - (e / m) * m;
-            if (i < j) {
-                const double aij = A[i * LA + j];
-                if (aij != 0.0 && fabs(aij) > JAC_EPS * sqrt(fabs(A[i * LA + i]) * fabs(A[j * LA + j]))) any = 1;
+    } else {
+        // update threads: fixed off-diagonal blocks (P < P2) and one slot pair of V
+        const int ut = c.tid - JW;
+        static_assert(JB <= JB_MAX, "block list");
+        int rd[JB], wr[JB][4], bP[JB], bQ[JB];
+#pragma unroll
+        for (int k = 0; k < JB; ++k) {
+            const int e = ut + k * JU;
+            bP[k] = -1; bQ[k] = 0; rd[k] = 0;
+            wr[k][0] = wr[k][1] = wr[k][2] = wr[k][3] = 0;
+            if (e < nb - np) {
+                int P = 0, start = 0, len = np - 1;
+                while (e >= start + len) { start += len; ++P; --len; }
+                const int P2 = P + 1 + (e - start), q = m - 1 - P, q2 = m - 1 - P2;
+                bP[k] = P; bQ[k] = P2; rd[k] = 2 * blk(P, P2);
+                wr[k][0] = eoff(nx(P), nx(P2)); wr[k][1] = eoff(nx(P), nx(q2));
+                wr[k][2] = eoff(nx(q), nx(P2)); wr[k][3] = eoff(nx(q), nx(q2));
             }
         }
-        if (any) *flag = 1;
+        const int vgroups = JU / np;
+        const int vS = ut % np, vG = ut / np;
+        const bool v_on = vG < vgroups;
         sync();
-        const int more = *flag;
-        sync();
-        if (!more) return sweep + 1;
+        for (int sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
+#pragma unroll 1
+            for (int round = 0; round < m - 1; ++round) {
+                const double* Ao = cur ? A1 : A0;
+                double* An = cur ? A0 : A1;
+                const double2* cso = reinterpret_cast<const double2*>(cur ? R1 : R0);
+                // Addresses are formed (and made opaque) while gathering; the stores use
+                // them as plain registers: an asm statement between stores would force a
+                // wait for every outstanding LDS operation.
+#pragma unroll
+                for (int k0 = 0; k0 < JB; k0 += JB_BATCH) {
+                    double2 xt[JB_BATCH], xb[JB_BATCH], r1[JB_BATCH], r2[JB_BATCH];
+                    int wa[JB_BATCH][4];
+#pragma unroll
+                    for (int u = 0; u < JB_BATCH; ++u) {
+                        const int k = k0 + u;
+                        if (k >= JB || bP[k] < 0) continue;
+                        const int o = opaque(rd[k]);
+                        xt[u] = *reinterpret_cast<const double2*>(Ao + o);
+                        xb[u] = *reinterpret_cast<const double2*>(Ao + 2 * nb + o);
+                        r1[u] = cso[opaque(bP[k])];
+                        r2[u] = cso[opaque(bQ[k])];
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) wa[u][v] = opaque(wr[k][v]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < JB_BATCH; ++u) {
+                        const int k = k0 + u;
+                        if (k >= JB || bP[k] < 0) continue;
+                        double z00, z01, z10, z11;
+                        jac_block(r1[u], r2[u], xt[u].x, xt[u].y, xb[u].x, xb[u].y, z00, z01, z10, z11);
+                        An[wa[u][0]] = z00; An[wa[u][1]] = z01; An[wa[u][2]] = z10; An[wa[u][3]] = z11;
+                    }
+                }
+                if (v_on) {
+                    const int s_ = opaque(vS), g_ = opaque(vG);   // row offsets stay in the loop
+                    const int vp = player(s_, round, m), vq = player(m - 1 - s_, round, m);
+                    const double2 rv = cso[s_];
+                    // two half batches of rows: all loads of a batch are issued before its stores
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        double vv[JV_HALF][2];
+                        int va[JV_HALF];
+#pragma unroll
+                        for (int k = 0; k < JV_HALF; ++k) {
+                            const int i = g_ + (h * JV_HALF + k) * vgroups;
+                            va[k] = i * LV;
+                            if (i < m) { vv[k][0] = V[va[k] + vp]; vv[k][1] = V[va[k] + vq]; }
+                        }
+#pragma unroll
+                        for (int k = 0; k < JV_HALF; ++k) {
+                            const int i = g_ + (h * JV_HALF + k) * vgroups;
+                            if (i < m) {
+                                V[va[k] + vp] = fma(rv.x, vv[k][0], -(rv.y * vv[k][1]));
+                                V[va[k] + vq] = fma(rv.y, vv[k][0], rv.x * vv[k][1]);
+                            }
+                        }
+                    }
+                }
+                sync();
+                cur ^= 1;
+            }
+            // convergence: does any pair of this thread's blocks exceed the threshold?
+            const double* Af = cur ? A1 : A0;
+            auto diag = [&](int x) {
+                const int bb = blk(pr(x), pr(x));
+                return x < np ? Af[2 * bb] : Af[2 * nb + 2 * bb + 1];
+            };
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < JB; ++k) {
+                if (bP[k] < 0) continue;
+                const int P = bP[k], P2 = bQ[k];
+                const double dP = diag(P), dq = diag(m - 1 - P), dP2 = diag(P2), dq2 = diag(m - 1 - P2);
+                any = any || big(Af[rd[k]], dP, dP2) || big(Af[rd[k] + 1], dP, dq2) ||
+                      big(Af[2 * nb + rd[k]], dq, dP2) || big(Af[2 * nb + rd[k] + 1], dq, dq2);
+            }
+            if (any) flag[sweep & 1] = 1;
+            sync();
+            if (!flag[sweep & 1]) { sweeps = sweep + 1; break; }
+        }
     }
-    return -1;
+    *fin = cur ? A1 : A0;
+    return sweeps;
 }
 
 // ---------------------------------------------------------------- blocked Cholesky / inverse
@@ -459,13 +590,6 @@ __device__ __forceinline__ int bidx(int I, int J) { return I * (I + 1) / 2 + J; 
 // root (hardware estimate plus two Newton steps) instead of a square root and a divide,
 // and the inverse is right-looking, so the dependent chains stay short. Returns false
 // on a non-positive pivot.
-__device__ __forceinline__ double rsqrt_refined(double d) {
-    double y = __builtin_amdgcn_rsq(d);
-    y = y * (1.5 - 0.5 * d * y * y);
-    y = y * (1.5 - 0.5 * d * y * y);
-    return y;
-}
-
 __device__ bool diag_factor_invert(double* B, int lane) {
     double row[16];
 #pragma unroll
@@ -651,6 +775,7 @@ __host__ __device__ __forceinline__ long long target_doubles(int mp) {
     return (long long)S_COUNT * mp * mp + mp + 16;
 }
 
+template <int JB, bool V_IN_LDS>
 __global__ __launch_bounds__(ET)
 void eigh_kernel(EighArgs a)
 {
@@ -666,14 +791,12 @@ void eigh_kernel(EighArgs a)
     double* extra = lam + mp;
     const double* Crb = a.Cr + (long long)b * a.sC_b;
 
-    const int LA = m + 1;     // odd row stride: the column-wise mirror writes hit distinct banks
-    double* Aj = lds + 48;
-    double* Vj = a.v_in_lds ? Aj + m * LA : ws + (long long)S_T5 * mp * mp;
-    const int LV = a.v_in_lds ? LA : mp;
-    double* tail = a.v_in_lds ? Vj + m * LA : Aj + m * LA;
-    int* pairs = reinterpret_cast<int*>(tail);            // m ints: ranks after the sweeps
-    double* rot = tail + m;                               // 3 m / 2 doubles
+    // LDS: [0, 48) reductions and flags, then the Jacobi area, then V when it fits
+    double* J = lds + 48;
+    const int np = m / 2, nb = np * (np + 1) / 2;
+    int* pairs = reinterpret_cast<int*>(J + 8 * nb + 8 * np);   // m ints: ranks after the sweeps
     int* flag = reinterpret_cast<int*>(c.red + FLAG_JACOBI);
+    const int LV = V_IN_LDS ? m + 1 : mp;                      // odd LDS row stride
 
     if (c.wave == 0) {
         double t = 0.0;
@@ -685,23 +808,38 @@ void eigh_kernel(EighArgs a)
     const double scale = fmax(c.red[40] / m, 1.0);
     for (int e = c.tid; e < m * m; e += ET) {
         const int i = e / m, j = e - (e / m) * m;
-        // lower triangle, as eigh(UPLO='L') reads it
-        Aj[i * LA + j] = Crb[(long long)max(i, j) * a.ldc + min(i, j)] / scale;
+        if (i <= j) {
+            // slot (= index) pair blocks of the upper triangle; Cr's lower triangle is read,
+            // as eigh(UPLO='L') reads it
+            const int pi = i < np ? i : m - 1 - i, hi = i >= np ? 1 : 0;
+            const int pj = j < np ? j : m - 1 - j, hj = j >= np ? 1 : 0;
+            int P = pi, P2 = pj, h1 = hi, h2 = hj;
+            if (P > P2 || (P == P2 && h1 > h2)) { P = pj; P2 = pi; h1 = hj; h2 = hi; }
+            const int blkidx = P * np - (P * (P - 1)) / 2 + (P2 - P);
+            J[(h1 ? 2 * nb : 0) + 2 * blkidx + h2] = Crb[(long long)j * a.ldc + i] / scale;
+        }
     }
     sync();
-    const int sweeps = jacobi<ET>(c, Aj, LA, Vj, LV, rot, flag);
+    const double* Af = nullptr;
+    double* Vj = V_IN_LDS ? J + jac_area(m) : ws + (long long)S_T5 * mp * mp;
+    const int sweeps = jacobi<JB, V_IN_LDS>(c, J, Vj, LV, flag, &Af);
+    auto eig = [&](int x) {
+        const int px = x < np ? x : m - 1 - x;
+        const int bi = 2 * (px * np - (px * (px - 1)) / 2);
+        return x < np ? Af[bi] : Af[2 * nb + bi + 1];
+    };
     for (int i = c.tid; i < m; i += ET) {
-        const double li = Aj[i * LA + i];
+        const double li = eig(i);
         int rank = 0;
         for (int j = 0; j < m; ++j) {
-            const double lj = Aj[j * LA + j];
+            const double lj = eig(j);
             rank += (lj < li || (lj == li && j < i)) ? 1 : 0;
         }
-        pairs[i] = rank;     // pairs[] is free after the sweeps
+        pairs[i] = rank;
     }
     sync();
     for (int i = c.tid; i < mp; i += ET) {
-        if (i < m) lam[pairs[i]] = Aj[i * LA + i];
+        if (i < m) lam[pairs[i]] = eig(i);
         else lam[i] = 1.0;
     }
     for (int e = c.tid; e < mp * mp; e += ET) {
@@ -713,7 +851,7 @@ void eigh_kernel(EighArgs a)
         extra[0] = scale;
         extra[1] = (double)sweeps;
         extra[2] = (double)(wall_clock64() - t0);
-        extra[3] = (double)a.v_in_lds;
+        extra[3] = V_IN_LDS ? 1.0 : 0.0;
     }
 }
 
@@ -1125,7 +1263,7 @@ size_t dual_workspace_doubles(int batch, int m) {
 }
 
 static size_t eigh_lds_doubles(int m, bool v_in_lds) {
-    return 48 + (size_t)m * (m + 1) * (v_in_lds ? 2 : 1) + 3 * (size_t)m + 8;
+    return 48 + (size_t)jac_area(m) + (v_in_lds ? (size_t)m * (m + 1) : 0);
 }
 
 static hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int ldc, int batch,
@@ -1136,10 +1274,16 @@ static hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC
     EighArgs e;
     e.Cr = Cr; e.sC_b = sC_b; e.ldc = ldc; e.m = m; e.mp = pad16(m); e.ws = ws;
     e.v_in_lds = v_in_lds ? 1 : 0;
-    hipError_t err = hipFuncSetAttribute((const void*)eigh_kernel,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int np = m / 2;
+    const bool small = (np * (np - 1) / 2 + JU - 1) / JU <= JB_BATCH;   // off-diagonal blocks per thread
+    const void* fn = small ? (v_in_lds ? (const void*)eigh_kernel<JB_BATCH, true> : (const void*)eigh_kernel<JB_BATCH, false>)
+                           : (v_in_lds ? (const void*)eigh_kernel<JB_MAX, true> : (const void*)eigh_kernel<JB_MAX, false>);
+    hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(eigh_kernel, dim3(batch), dim3(ET), lds, stream, e);
+    if (small && v_in_lds) hipLaunchKernelGGL((eigh_kernel<JB_BATCH, true>), dim3(batch), dim3(ET), lds, stream, e);
+    else if (small) hipLaunchKernelGGL((eigh_kernel<JB_BATCH, false>), dim3(batch), dim3(ET), lds, stream, e);
+    else if (v_in_lds) hipLaunchKernelGGL((eigh_kernel<JB_MAX, true>), dim3(batch), dim3(ET), lds, stream, e);
+    else hipLaunchKernelGGL((eigh_kernel<JB_MAX, false>), dim3(batch), dim3(ET), lds, stream, e);
     return hipGetLastError();
 }
 
