@@ -336,7 +336,8 @@ constexpr int JW = 64;        // wave 0: diagonal blocks and rotation parameters
 constexpr int JU = 512 - JW;  // threads that update off-diagonal blocks and V
 constexpr int JB_MAX = 5;     // off-diagonal blocks per update thread: ceil(64*63/2 / 448)
 constexpr int JB_BATCH = 3;   // blocks gathered before their stores
-constexpr int JV_HALF = 10;   // eigenvector rows per update thread: 2 x 10 >= ceil(128 / (448 / 64))
+constexpr int JV_SMALL = 11;  // eigenvector rows per update thread, one batch (kernel for m <= 98)
+constexpr int JV_HALF = 10;   // rows per batch, two batches: 20 >= ceil(128 / (448 / 64))
 constexpr double JAC_EPS = 2.220446049250313e-16;
 
 __host__ __device__ __forceinline__ int jac_blocks(int m) { return (m / 2) * (m / 2 + 1) / 2; }
@@ -378,7 +379,7 @@ __device__ __forceinline__ void jac_block(double2 r1, double2 r2, double x00, do
 // player-ordered V (m x m, row stride LV, initialised here). Returns the sweep count
 // (-1 without convergence) and in `fin` the buffer holding the final A.
 template <int JB, bool V_IN_LDS>
-__device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int* flag, const double** fin) {
+__device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, int* flag, const double** fin) {
     const int m = c.m, np = m / 2, nb = np * (np + 1) / 2;
     double* A0 = J;
     double* A1 = J + 4 * nb;
@@ -493,6 +494,7 @@ __device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int* flag, con
         const int vgroups = JU / np;
         const int vS = ut % np, vG = ut / np;
         const bool v_on = vG < vgroups;
+        constexpr int JV = JB <= JB_BATCH ? JV_SMALL : JV_HALF, NVB = JB <= JB_BATCH ? 1 : 2;
         sync();
         for (int sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
 #pragma unroll 1
@@ -532,24 +534,22 @@ __device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int* flag, con
                     const int s_ = opaque(vS), g_ = opaque(vG);   // row offsets stay in the loop
                     const int vp = player(s_, round, m), vq = player(m - 1 - s_, round, m);
                     const double2 rv = cso[s_];
-                    // two half batches of rows: all loads of a batch are issued before its stores
+                    // batches of rows: all loads of a batch are issued before its stores;
+                    // rows past m use the dummy row (no branch)
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        double vv[JV_HALF][2];
-                        int va[JV_HALF];
+                    for (int h = 0; h < NVB; ++h) {
+                        double vv[JV][2];
+                        int va[JV];
 #pragma unroll
-                        for (int k = 0; k < JV_HALF; ++k) {
-                            const int i = g_ + (h * JV_HALF + k) * vgroups;
-                            va[k] = i * LV;
-                            if (i < m) { vv[k][0] = V[va[k] + vp]; vv[k][1] = V[va[k] + vq]; }
+                        for (int k = 0; k < JV; ++k) {
+                            const int i = g_ + (h * JV + k) * vgroups;
+                            va[k] = i < m ? (int)__umul24(i, LV) : vdummy;
+                            vv[k][0] = V[va[k] + vp]; vv[k][1] = V[va[k] + vq];
                         }
 #pragma unroll
-                        for (int k = 0; k < JV_HALF; ++k) {
-                            const int i = g_ + (h * JV_HALF + k) * vgroups;
-                            if (i < m) {
-                                V[va[k] + vp] = fma(rv.x, vv[k][0], -(rv.y * vv[k][1]));
-                                V[va[k] + vq] = fma(rv.y, vv[k][0], rv.x * vv[k][1]);
-                            }
+                        for (int k = 0; k < JV; ++k) {
+                            V[va[k] + vp] = fma(rv.x, vv[k][0], -(rv.y * vv[k][1]));
+                            V[va[k] + vq] = fma(rv.y, vv[k][0], rv.x * vv[k][1]);
                         }
                     }
                 }
@@ -821,8 +821,10 @@ void eigh_kernel(EighArgs a)
     }
     sync();
     const double* Af = nullptr;
+    // V's dummy row (target of row indices past m): one extra LDS row, or slot T4 below T5
     double* Vj = V_IN_LDS ? J + jac_area(m) : ws + (long long)S_T5 * mp * mp;
-    const int sweeps = jacobi<JB, V_IN_LDS>(c, J, Vj, LV, flag, &Af);
+    const int vdummy = V_IN_LDS ? m * LV : -(mp * mp);
+    const int sweeps = jacobi<JB, V_IN_LDS>(c, J, Vj, LV, vdummy, flag, &Af);
     auto eig = [&](int x) {
         const int px = x < np ? x : m - 1 - x;
         const int bi = 2 * (px * np - (px * (px - 1)) / 2);
@@ -1263,7 +1265,7 @@ size_t dual_workspace_doubles(int batch, int m) {
 }
 
 static size_t eigh_lds_doubles(int m, bool v_in_lds) {
-    return 48 + (size_t)jac_area(m) + (v_in_lds ? (size_t)m * (m + 1) : 0);
+    return 48 + (size_t)jac_area(m) + (v_in_lds ? (size_t)(m + 1) * (m + 1) : 0);   // V and its dummy row
 }
 
 static hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int ldc, int batch,
@@ -1275,7 +1277,10 @@ static hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC
     e.Cr = Cr; e.sC_b = sC_b; e.ldc = ldc; e.m = m; e.mp = pad16(m); e.ws = ws;
     e.v_in_lds = v_in_lds ? 1 : 0;
     const int np = m / 2;
-    const bool small = (np * (np - 1) / 2 + JU - 1) / JU <= JB_BATCH;   // off-diagonal blocks per thread
+    const int blocks = (np * (np - 1) / 2 + JU - 1) / JU;          // off-diagonal blocks per thread
+    const int rows = (m + JU / np - 1) / (JU / np);                 // eigenvector rows per thread
+    const bool small = blocks <= JB_BATCH && rows <= JV_SMALL;
+    if (blocks > JB_MAX || rows > 2 * JV_HALF) return hipErrorInvalidValue;
     const void* fn = small ? (v_in_lds ? (const void*)eigh_kernel<JB_BATCH, true> : (const void*)eigh_kernel<JB_BATCH, false>)
                            : (v_in_lds ? (const void*)eigh_kernel<JB_MAX, true> : (const void*)eigh_kernel<JB_MAX, false>);
     hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
