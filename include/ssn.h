@@ -98,6 +98,23 @@ int ssn_solve(void* stream, const double* T, long long sT_b, long long sT_r, int
               double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
               long long* iters, int* status, int* counters, void* ws, size_t ws_bytes);
 
+/* Split-precision forms of ssn_gram, ssn_recover and ssn_solve (same arguments and workspace).
+ * No reference counterpart (the reference's Gram and recovery are FP64 only,
+ * gram_triangle_reflect.cu:1-104, recovery_fused.cu:1-41). Every FP64 operand of the Gram and
+ * of the recovery is cut into 5 BF16 slices under a power-of-two scale shared by its group
+ * (a column of [A | D] per row chunk for the Gram; a row of [A | D], and a column of the
+ * recovery matrix, for the recovery); slice products run on the BF16 MFMA with exact FP32
+ * accumulation per order and are reassembled in FP64. The dual phase is FP64 as in ssn_dual.
+ * Relative error of Cr, Ci and X is about 2^-39 of the group maxima. */
+int ssn_gram_split(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+                   double* Cr, double* Ci, long long sC_b, int ldc, void* ws, size_t ws_bytes);
+int ssn_recover_split(void* stream, const double* T, long long sT_b, long long sT_r, const double* P,
+                      const double* Q, int batch, int n, int m, double* X, long long sX_b, long long sX_r,
+                      void* ws, size_t ws_bytes);
+int ssn_solve_split(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+                    double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
+                    long long* iters, int* status, int* counters, void* ws, size_t ws_bytes);
+
 /* Backward of the projection (no counterpart in the reference, which is forward-only): for the
  * upstream gradient G = dL/dX ([batch][2n][m], strides sG_b, sG_r) writes dT = dL/dT. `ws` must be
  * the workspace of the ssn_solve call that produced X, unmodified since. Implicit differentiation

@@ -39,6 +39,9 @@ SIGNATURES = {
     "ssn_recover": (_I, [_P, _P, _L, _L, _P, _P, _I, _I, _I, _P, _L, _L, _P, _S]),
     "ssn_solve": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _L, _I, _P, _P, _P, _P, _S]),
     "ssn_backward": (_I, [_P, _P, _L, _L, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _P, _S]),
+    "ssn_gram_split": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _P, _L, _I, _P, _S]),
+    "ssn_recover_split": (_I, [_P, _P, _L, _L, _P, _P, _I, _I, _I, _P, _L, _L, _P, _S]),
+    "ssn_solve_split": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _L, _I, _P, _P, _P, _P, _S]),
 }
 
 _lib = None

@@ -13,7 +13,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libssn.so"
-SOURCES = ["gram.hip", "recover.hip", "dual.hip", "capi.hip"]
+SOURCES = ["gram.hip", "recover.hip", "split.hip", "dual.hip", "capi.hip"]
 ARCH = os.environ.get("SSN_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -26,6 +26,10 @@ hipError_t launch_backward_stream(hipStream_t, const double*, long long, long lo
 long long workspace_target_doubles(int m);
 hipError_t launch_dual_newton(hipStream_t, const double*, const double*, long long, int, int, int,
                               double*, long long, int, double*, double*, long long*, int*, int*, double*, size_t);
+hipError_t launch_split_gram(hipStream_t, const double*, long long, long long, int, int, int, int, double*);
+long long split_fragment_doubles(int m);
+hipError_t launch_split_recover(hipStream_t, const double*, long long, long long, const double*, const double*,
+                                double*, int, int, int, double*, long long, long long);
 }  // namespace ssn
 
 using namespace ssn;
@@ -125,8 +129,8 @@ size_t ssn_workspace_bytes(int batch, int n, int m) {
     return layout(batch, n, m).total * sizeof(double);
 }
 
-int ssn_gram(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
-             double* Cr, double* Ci, long long sC_b, int ldc, void* ws, size_t ws_bytes)
+static int gram_impl(bool split, void* stream, const double* T, long long sT_b, long long sT_r, int batch,
+                     int n, int m, double* Cr, double* Ci, long long sC_b, int ldc, void* ws, size_t ws_bytes)
 {
     if (!shape_ok(batch, n, m) || !T || !Cr || !Ci || ldc < m) return SSN_ERR_ARGS;
     if (batch == 0) return SSN_OK;
@@ -137,9 +141,22 @@ int ssn_gram(void* stream, const double* T, long long sT_b, long long sT_r, int 
         return SSN_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     double* part = (double*)ws + L.partial;
-    rc = hip_status(launch_gram_partial(s, T, sT_b, sT_r, batch, n, m, L.chunks, part));
+    rc = hip_status(split ? launch_split_gram(s, T, sT_b, sT_r, batch, n, m, L.chunks, part)
+                          : launch_gram_partial(s, T, sT_b, sT_r, batch, n, m, L.chunks, part));
     if (rc) return rc;
     return hip_status(launch_gram_reduce(s, part, batch, L.chunks, m, Cr, Ci, sC_b, ldc));
+}
+
+int ssn_gram(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+             double* Cr, double* Ci, long long sC_b, int ldc, void* ws, size_t ws_bytes)
+{
+    return gram_impl(false, stream, T, sT_b, sT_r, batch, n, m, Cr, Ci, sC_b, ldc, ws, ws_bytes);
+}
+
+int ssn_gram_split(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+                   double* Cr, double* Ci, long long sC_b, int ldc, void* ws, size_t ws_bytes)
+{
+    return gram_impl(true, stream, T, sT_b, sT_r, batch, n, m, Cr, Ci, sC_b, ldc, ws, ws_bytes);
 }
 
 int ssn_dual(void* stream, const double* Cr, const double* Ci, long long sC_b, int ldc, int batch, int m,
@@ -200,9 +217,23 @@ int ssn_recover(void* stream, const double* T, long long sT_b, long long sT_r, c
     return hip_status(launch_recover(s, T, sT_b, sT_r, (const double*)ws, batch, n, m, X, sX_b, sX_r));
 }
 
-int ssn_solve(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
-              double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
-              long long* iters, int* status, int* counters, void* ws, size_t ws_bytes)
+int ssn_recover_split(void* stream, const double* T, long long sT_b, long long sT_r, const double* P,
+                      const double* Q, int batch, int n, int m, double* X, long long sX_b, long long sX_r,
+                      void* ws, size_t ws_bytes)
+{
+    if (!shape_ok(batch, n, m) || !T || !P || !Q || !X) return SSN_ERR_ARGS;
+    if (batch == 0) return SSN_OK;
+    int rc = current_device_ready(nullptr);
+    if (rc) return rc;
+    const size_t need = (size_t)batch * split_fragment_doubles(m) * sizeof(double);
+    if (!ws || ws_bytes < need) return SSN_ERR_WORKSPACE;
+    return hip_status(launch_split_recover((hipStream_t)stream, T, sT_b, sT_r, P, Q, (double*)ws, batch, n, m,
+                                           X, sX_b, sX_r));
+}
+
+static int solve_impl(bool split, void* stream, const double* T, long long sT_b, long long sT_r, int batch,
+                      int n, int m, double* X, long long sX_b, long long sX_r, double* omega, long long sO_b,
+                      int ldo, long long* iters, int* status, int* counters, void* ws, size_t ws_bytes)
 {
     if (!shape_ok(batch, n, m) || !T || !X || !omega || !iters || !status || ldo < m) return SSN_ERR_ARGS;
     if (batch == 0) return SSN_OK;
@@ -213,7 +244,8 @@ int ssn_solve(void* stream, const double* T, long long sT_b, long long sT_r, int
     if (!ws || ws_bytes < L.total * sizeof(double)) return SSN_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     double* w = (double*)ws;
-    rc = hip_status(launch_gram_partial(s, T, sT_b, sT_r, batch, n, m, L.chunks, w + L.partial));
+    rc = hip_status(split ? launch_split_gram(s, T, sT_b, sT_r, batch, n, m, L.chunks, w + L.partial)
+                          : launch_gram_partial(s, T, sT_b, sT_r, batch, n, m, L.chunks, w + L.partial));
     if (rc) return rc;
     rc = hip_status(launch_gram_reduce(s, w + L.partial, batch, L.chunks, m, w + L.cr, w + L.ci,
                                        (long long)m * m, m));
@@ -221,9 +253,28 @@ int ssn_solve(void* stream, const double* T, long long sT_b, long long sT_r, int
     rc = hip_status(launch_dual(s, w + L.cr, w + L.ci, (long long)m * m, m, batch, m, omega, sO_b, ldo,
                                 w + L.pf, w + L.qf, iters, status, counters, w + L.dual, lds));
     if (rc) return rc;
+    if (split)
+        return hip_status(launch_split_recover(s, T, sT_b, sT_r, w + L.pf, w + L.qf, w + L.frag, batch, n, m,
+                                               X, sX_b, sX_r));
     rc = hip_status(launch_recovery_fragments(s, w + L.pf, w + L.qf, batch, m, w + L.frag));
     if (rc) return rc;
     return hip_status(launch_recover(s, T, sT_b, sT_r, w + L.frag, batch, n, m, X, sX_b, sX_r));
+}
+
+int ssn_solve(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+              double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
+              long long* iters, int* status, int* counters, void* ws, size_t ws_bytes)
+{
+    return solve_impl(false, stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo, iters, status,
+                      counters, ws, ws_bytes);
+}
+
+int ssn_solve_split(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+                    double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
+                    long long* iters, int* status, int* counters, void* ws, size_t ws_bytes)
+{
+    return solve_impl(true, stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo, iters, status,
+                      counters, ws, ws_bytes);
 }
 
 int ssn_backward(void* stream, const double* T, long long sT_b, long long sT_r,

@@ -78,11 +78,13 @@ def _workspace(nbytes, device):
     return torch.empty(max(int(nbytes), 8), dtype=torch.uint8, device=device)
 
 
-def _solve_device_ws(T, check=True):
-    """Device solve that also returns the workspace holding the forward state (for backward)."""
+def _run_solve(T, split):
+    """One ssn_solve / ssn_solve_split call on device T; returns outputs, workspace and stats."""
     lib = _require_ready()
     batch, n, m = _check_targets(T)
     dev = T.device
+    if T.stride(-1) != 1:
+        T = T.contiguous()
     X = torch.empty((batch, 2 * n, m), dtype=torch.float64, device=dev)
     omega = torch.empty((batch, m, m), dtype=torch.float64, device=dev)
     iters = torch.zeros(batch, dtype=torch.int64, device=dev)
@@ -90,16 +92,23 @@ def _solve_device_ws(T, check=True):
     counters = torch.zeros((batch, 12), dtype=torch.int32, device=dev)
     ws = _workspace(lib.ssn_workspace_bytes(batch, n, m) if batch else 8, dev)
     if batch:
+        fn, name = (lib.ssn_solve_split, "ssn_solve_split") if split else (lib.ssn_solve, "ssn_solve")
         with torch.cuda.device(dev):
-            rc = lib.ssn_solve(_stream(), T.data_ptr(), T.stride(0), T.stride(1), batch, n, m,
-                               X.data_ptr(), X.stride(0), X.stride(1),
-                               omega.data_ptr(), omega.stride(0), omega.stride(1),
-                               iters.data_ptr(), status.data_ptr(), counters.data_ptr(),
-                               ws.data_ptr(), ws.numel())
-        _lib.check(rc, "ssn_solve")
-    if check:
-        _raise_for_status(status)
+            rc = fn(_stream(), T.data_ptr(), T.stride(0), T.stride(1), batch, n, m,
+                    X.data_ptr(), X.stride(0), X.stride(1),
+                    omega.data_ptr(), omega.stride(0), omega.stride(1),
+                    iters.data_ptr(), status.data_ptr(), counters.data_ptr(),
+                    ws.data_ptr(), ws.numel())
+        _lib.check(rc, name)
     return X, omega, iters, ws, dict(status=status, counters=counters)
+
+
+def _solve_device_ws(T, check=True, split=False):
+    """Device solve that also returns the workspace holding the forward state (for backward)."""
+    X, omega, iters, ws, stats = _run_solve(T, split)
+    if check:
+        _raise_for_status(stats["status"])
+    return X, omega, iters, ws, stats
 
 
 def backward_device(T, G, ws, return_stats=False):
@@ -126,9 +135,9 @@ class _Projection(torch.autograd.Function):
     """X = P(T) with a backward by implicit differentiation of the dual optimality condition."""
 
     @staticmethod
-    def forward(ctx, T):
+    def forward(ctx, T, split):
         T = T.contiguous()
-        X, omega, iters, ws, _ = _solve_device_ws(T)
+        X, omega, iters, ws, _ = _solve_device_ws(T, split=split)
         ctx.save_for_backward(T)
         ctx.ws = ws
         ctx.mark_non_differentiable(omega, iters)
@@ -138,75 +147,60 @@ class _Projection(torch.autograd.Function):
     def backward(ctx, gX, gomega, giters):
         (T,) = ctx.saved_tensors
         if gX is None:
-            return None
-        return backward_device(T, gX, ctx.ws)
+            return None, None
+        return backward_device(T, gX, ctx.ws), None
 
 
-def project(T):
+def project(T, split=False):
     """Differentiable projection: (X, omega, iterations) = solve(T) with dX/dT available to autograd.
 
     Gradients flow from X only (omega and the iteration counts are non-differentiable outputs).
     Device input runs in place; CPU input is moved to the current device and the outputs back.
+    split=True runs the forward's Gram and recovery as split BF16 products (see solve); the
+    backward is FP64 in both cases.
     """
     _require_ready()
     _check_targets(T)
     if T.is_cuda:
-        return _Projection.apply(T)
+        return _Projection.apply(T, bool(split))
     dev = torch.device("cuda", torch.cuda.current_device())
-    X, omega, iters = _Projection.apply(T.to(dev))
+    X, omega, iters = _Projection.apply(T.to(dev), bool(split))
     return X.cpu(), omega.cpu(), iters.cpu()
 
 
 @torch.no_grad()
-def solve_device(T, return_stats=False, check=True):
+def solve_device(T, return_stats=False, check=True, split=False):
     """Device-resident solve: T is a float64 HIP tensor [batch, 2n, 2k]; outputs stay on the device.
 
     Returns (X, omega, iterations[, stats]) where stats = dict(status, counters[batch, 12]
     with columns newton, cg, backtracks, hvps, Jacobi sweeps, 0, then 10 ns device ticks of
     eigensolver, setup, CG, line search, whole kernel, 0).
+    split=True computes the Gram and the recovery as split BF16 products with FP32
+    accumulation reassembled to FP64 (every other step in FP64).
     """
-    lib = _require_ready()
-    batch, n, m = _check_targets(T)
     assert T.is_cuda, "solve_device expects a device tensor"
-    dev = T.device
-    if T.stride(-1) != 1:
-        T = T.contiguous()
-    X = torch.empty((batch, 2 * n, m), dtype=torch.float64, device=dev)
-    omega = torch.empty((batch, m, m), dtype=torch.float64, device=dev)
-    iters = torch.zeros(batch, dtype=torch.int64, device=dev)
-    status = torch.zeros(batch, dtype=torch.int32, device=dev)
-    counters = torch.zeros((batch, 12), dtype=torch.int32, device=dev)
-    if batch:
-        nbytes = lib.ssn_workspace_bytes(batch, n, m)
-        ws = _workspace(nbytes, dev)
-        with torch.cuda.device(dev):
-            rc = lib.ssn_solve(_stream(), T.data_ptr(), T.stride(0), T.stride(1), batch, n, m,
-                               X.data_ptr(), X.stride(0), X.stride(1),
-                               omega.data_ptr(), omega.stride(0), omega.stride(1),
-                               iters.data_ptr(), status.data_ptr(), counters.data_ptr(),
-                               ws.data_ptr(), ws.numel())
-        _lib.check(rc, "ssn_solve")
+    X, omega, iters, _, stats = _run_solve(T, split)
     if check:
-        _raise_for_status(status)
+        _raise_for_status(stats["status"])
     if return_stats:
-        return X, omega, iters, dict(status=status, counters=counters)
+        return X, omega, iters, stats
     return X, omega, iters
 
 
 @torch.no_grad()
-def solve(T, return_stats=False):
+def solve(T, return_stats=False, split=False):
     """(X, omega, iterations) for float64 targets T [batch, 2n, 2k] (dual_recommended_hvp.py:231-242).
 
     CPU input -> CPU outputs (H2D and D2H included, X pinned when >= 64 MiB as in
-    dual_recommended_hvp.py:204-205); device input -> device outputs.
+    dual_recommended_hvp.py:204-205); device input -> device outputs. split: see solve_device.
     """
     _require_ready()
     _check_targets(T)
     if T.is_cuda:
-        return solve_device(T, return_stats=return_stats)
+        return solve_device(T, return_stats=return_stats, split=split)
     dev = torch.device("cuda", torch.cuda.current_device())
     Td = T.to(dev)
-    out = solve_device(Td, return_stats=return_stats)
+    out = solve_device(Td, return_stats=return_stats, split=split)
     X, omega, iters = out[:3]
     direct = X.numel() * X.element_size() >= PINNED_OUTPUT_MIN_BYTES
     Xh = torch.empty(X.shape, dtype=X.dtype, pin_memory=direct)
@@ -221,8 +215,10 @@ def solve(T, return_stats=False):
 # ---------------------------------------------------------------- the three stages, separately
 
 @torch.no_grad()
-def gram_chunks(T):
-    """Device Cr = A^T A + D^T D, Ci = A^T D - D^T A for device T (dual_recommended_hvp.py:79-96)."""
+def gram_chunks(T, split=False):
+    """Device Cr = A^T A + D^T D, Ci = A^T D - D^T A for device T (dual_recommended_hvp.py:79-96).
+
+    split=True: split BF16 products, FP32 accumulation, FP64 reassembly."""
     lib = _require_ready()
     batch, n, m = _check_targets(T)
     if T.stride(-1) != 1:
@@ -231,10 +227,11 @@ def gram_chunks(T):
     Ci = torch.empty_like(Cr)
     if batch:
         ws = _workspace(lib.ssn_workspace_bytes(batch, n, m), T.device)
-        rc = lib.ssn_gram(_stream(), T.data_ptr(), T.stride(0), T.stride(1), batch, n, m,
-                          Cr.data_ptr(), Ci.data_ptr(), Cr.stride(0), Cr.stride(1),
-                          ws.data_ptr(), ws.numel())
-        _lib.check(rc, "ssn_gram")
+        fn = lib.ssn_gram_split if split else lib.ssn_gram
+        rc = fn(_stream(), T.data_ptr(), T.stride(0), T.stride(1), batch, n, m,
+                Cr.data_ptr(), Ci.data_ptr(), Cr.stride(0), Cr.stride(1),
+                ws.data_ptr(), ws.numel())
+        _lib.check(rc, "ssn_gram_split" if split else "ssn_gram")
     return Cr, Ci
 
 
@@ -317,8 +314,10 @@ def dual_batch_staged(Cr, Ci, events=None, check=True):
 
 
 @torch.no_grad()
-def recover_matmul(T, P, Q):
-    """X = [A P^T + D Q^T; D P^T - A Q^T] on device (dual_recommended_hvp.py:194-228)."""
+def recover_matmul(T, P, Q, split=False):
+    """X = [A P^T + D Q^T; D P^T - A Q^T] on device (dual_recommended_hvp.py:194-228).
+
+    split=True: split BF16 products, FP32 accumulation, FP64 reassembly."""
     lib = _require_ready()
     batch, n, m = _check_targets(T)
     mp = lib.ssn_padded_order(m)
@@ -327,8 +326,9 @@ def recover_matmul(T, P, Q):
         T = T.contiguous()
     X = torch.empty((batch, 2 * n, m), dtype=torch.float64, device=T.device)
     if batch:
-        ws = _workspace(batch * 4 * mp * mp * 8, T.device)
-        rc = lib.ssn_recover(_stream(), T.data_ptr(), T.stride(0), T.stride(1), P.data_ptr(), Q.data_ptr(),
-                             batch, n, m, X.data_ptr(), X.stride(0), X.stride(1), ws.data_ptr(), ws.numel())
-        _lib.check(rc, "ssn_recover")
+        ws = _workspace(batch * 8 * mp * mp * 8, T.device)
+        fn = lib.ssn_recover_split if split else lib.ssn_recover
+        rc = fn(_stream(), T.data_ptr(), T.stride(0), T.stride(1), P.data_ptr(), Q.data_ptr(),
+                batch, n, m, X.data_ptr(), X.stride(0), X.stride(1), ws.data_ptr(), ws.numel())
+        _lib.check(rc, "ssn_recover_split" if split else "ssn_recover")
     return X
