@@ -2,7 +2,10 @@
 
 Workload (one "step"): one batched solve of 64 seeded targets with n = 4096, k = 48
 (T of shape 8192 x 96, FP64, entries N(0, 1/(2n)), seeds as the reference experiment),
-inputs already resident in HBM. Prints ONE JSON line; see DESIGN.md for the figures.
+inputs already resident in HBM, all arithmetic FP64. Beside it: the same step with the
+Gram and recovery as split BF16 products, the per-target loop, one C call, the
+CPU-in/CPU-out boundary, forward+backward through the autograd projection (FP64 and
+split forward) and the CPU oracle. Prints ONE JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -84,37 +87,40 @@ def main():
     m = 2 * K
     log(f"inputs ready: {BATCH} x {tuple(T.shape[1:])}")
 
-    def step(ev=None):
+    stream = torch.cuda.current_stream()
+
+    def step(split, ev=None):
         if ev:
             ev[0].record(stream)
-        Cr, Ci = ssn.gram_chunks(T)
+        Cr, Ci = ssn.gram_chunks(T, split=split)
         omega, P, Q, iters, st = ssn.dual_batch_staged(Cr, Ci, events=ev[1:4] if ev else None, check=False)
-        X = ssn.recover_matmul(T, P, Q)
+        X = ssn.recover_matmul(T, P, Q, split=split)
         if ev:
             ev[4].record(stream)
         return X, omega, iters, st
 
-    stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    log("warm-up done")
+    def timed(split):
+        """K timed steps bracketed by device synchronisation; per-stage HIP event averages."""
+        for _ in range(args.warmup):
+            step(split)
+        torch.cuda.synchronize()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s_ in range(args.steps):
+            out = step(split, ev[s_])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st = {name: sum(e[i].elapsed_time(e[i + 1]) for e in ev) / args.steps
+              for i, name in enumerate(("gram", "eigh", "newton", "recover"))}
+        return out, wall * 1e3 / args.steps, st
 
-    # timed region: K steps; HIP events on the stream every kernel is launched on
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s_ in range(args.steps):
-        X, omega, iters, st = step(ev[s_])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gram_ms = sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps
-    eigh_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps
-    newton_ms = sum(e[2].elapsed_time(e[3]) for e in ev) / args.steps
-    rec_ms = sum(e[3].elapsed_time(e[4]) for e in ev) / args.steps
-    ms_per_step = wall * 1e3 / args.steps
-    log(f"batched: {ms_per_step:.3f} ms/step (gram {gram_ms:.3f}, eigh {eigh_ms:.3f}, "
-        f"newton {newton_ms:.3f}, recover {rec_ms:.3f})")
+    # headline: the batched FP64 solve; beside it the batched split-precision solve
+    (X, omega, iters, st), ms_per_step, stages = timed(False)
+    log(f"batched fp64:  {ms_per_step:.3f} ms/step " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()))
+    (Xs, _, _, sts), ms_split, stages_split = timed(True)
+    log(f"batched split: {ms_split:.3f} ms/step " + ", ".join(f"{k} {v:.3f}" for k, v in stages_split.items()))
+    gram_ms, eigh_ms, newton_ms, rec_ms = (stages[k] for k in ("gram", "eigh", "newton", "recover"))
 
     status = st["status"].cpu()
     counters = st["counters"].cpu()
@@ -122,7 +128,11 @@ def main():
     met = acceptance.inspect(T, X)
     acc = acceptance.summary(met)
     acc["status_ok"] = ok_status
-    log("acceptance", acc)
+    acc_split = acceptance.summary(acceptance.inspect(T, Xs))
+    acc_split["status_ok"] = bool((sts["status"] == 0).all().item())
+    split_vs_fp64 = float(((Xs - X).norm() / X.norm()).item())
+    log("acceptance fp64", acc, "split", acc_split, f"split vs fp64 X: {split_vs_fp64:.2e}")
+    del Xs
 
     # the same solve through the single C call
     torch.cuda.synchronize()
@@ -132,7 +142,7 @@ def main():
     torch.cuda.synchronize()
     fused_ms = (time.perf_counter() - t1) * 1e3 / 3
 
-    # plain: per-target loop (experiment.py:75-80), device-resident
+    # plain: per-target loop of the FP64 solver (experiment.py:75-80), device-resident
     plain_ms = None
     if args.plain_reps > 0:
         runs = []
@@ -151,24 +161,24 @@ def main():
     G = inputs.make_gradients(N, K, BATCH).to(dev)
     Tg = T.clone().requires_grad_(True)
 
-    def fwd_bwd():
+    def fwd_bwd(split):
         Tg.grad = None
-        X_, _, _ = ssn.project(Tg)
+        X_, _, _ = ssn.project(Tg, split=split)
         torch.autograd.backward(X_, G)
         return Tg.grad
 
-    for _ in range(2):
-        fwd_bwd()
-    torch.cuda.synchronize()
-    fb_reps = 5
-    e0 = [torch.cuda.Event(enable_timing=True) for _ in range(fb_reps + 1)]
-    t4 = time.perf_counter()
-    for i in range(fb_reps):
-        e0[i].record(stream)
-        fwd_bwd()
-    e0[fb_reps].record(stream)
-    torch.cuda.synchronize()
-    fb_ms = (time.perf_counter() - t4) * 1e3 / fb_reps
+    fb = {}
+    for split in (True, False):
+        for _ in range(2):
+            fwd_bwd(split)
+        torch.cuda.synchronize()
+        fb_reps = 5
+        t4 = time.perf_counter()
+        for i in range(fb_reps):
+            fwd_bwd(split)
+        torch.cuda.synchronize()
+        fb[split] = (time.perf_counter() - t4) * 1e3 / fb_reps
+    fb_ms = fb[False]
     # backward alone (same forward state), HIP events on the launch stream
     X_, om_, it_, ws_, _ = ssn.dual._solve_device_ws(T)
     eb = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -183,7 +193,8 @@ def main():
     adj_it = bst["counters"][:, 0].double().mean().item()
     adj_ok = bool((bst["counters"][:, 1] == 0).all().item())
     del X_, om_, it_, ws_, dT, Tg
-    log(f"fwd+bwd: {fb_ms:.3f} ms (backward alone {bwd_ms:.3f} ms, adjoint CG {adj_it:.1f} it)")
+    log(f"fwd+bwd: fp64 {fb[False]:.3f} ms, split forward {fb[True]:.3f} ms (backward alone {bwd_ms:.3f} ms, "
+        f"adjoint CG {adj_it:.1f} it)")
 
     # reference protocol boundary: CPU tensor in -> CPU tensors out (PCIe included)
     ssn.solve(T_cpu)
@@ -196,23 +207,34 @@ def main():
 
     fd = dual_flops(m, counters)
     f_eigh = 9.0 * m ** 3 * BATCH               # dense symmetric eigensolver with vectors
-    f_gram = 8.0 * N * m * m * BATCH
+    f_gram = 8.0 * N * m * m * BATCH            # full Cr, Ci products (FP64-equivalent)
     f_rec = 8.0 * N * m * m * BATCH
-    f_stream = f_gram + f_rec
+    b_gram = 16.0 * N * m * BATCH               # read T once
+    b_rec = 32.0 * N * m * BATCH                # read T, write X
     bytes_fwd = 48.0 * N * m * BATCH
 
-    def roof(kernel, flops, ms):
+    def roof_mfma(kernel, flops, ms):
         achieved = flops / (ms * 1e-3) / 1e12
         return {"kernel": kernel, "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                 "algorithmic_flops_per_launch": flops, "avg_ms": ms}
 
-    kernels = [roof("ssn::gram_partial_kernel (+ gram_reduce_kernel)", f_gram, gram_ms),
-               roof("ssn::eigh_kernel", f_eigh, eigh_ms),
-               roof("ssn::dual_kernel", fd, newton_ms),
-               roof("ssn::stream_gemm_kernel<6,1> (+ build_recovery_fragments)", f_rec, rec_ms)]
+    def roof_hbm(kernel, nbytes, flops, ms):
+        achieved = nbytes / (ms * 1e-3) / 1e9
+        return {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "algorithmic_bytes_per_launch": nbytes,
+                "fp64_equivalent_tflops": flops / (ms * 1e-3) / 1e12, "avg_ms": ms}
+
+    kernels = [roof_mfma("ssn::gram_partial_kernel<6> (+ gram_reduce_kernel)", f_gram, gram_ms),
+               roof_mfma("ssn::eigh_kernel<3,true>", f_eigh, eigh_ms),
+               roof_mfma("ssn::dual_kernel<6>", fd, newton_ms),
+               roof_mfma("ssn::stream_gemm_kernel<6,1> (+ build_recovery_fragments)", f_rec, rec_ms)]
+    kernels_split = [roof_hbm("ssn::split_gram_kernel<6,2,5> (+ gram_reduce_kernel)", b_gram, f_gram,
+                              stages_split["gram"]),
+                     roof_hbm("ssn::split_recover_kernel<6,5> (+ split_recovery_fragments)", b_rec, f_rec,
+                              stages_split["recover"])]
     dominant = max(kernels, key=lambda k: k["avg_ms"])
-    stages = {"gram": gram_ms, "eigh": eigh_ms, "newton": newton_ms, "recover": rec_ms}
 
     out = {
         "metric": METRIC,
@@ -227,8 +249,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain FP64 Dual, batched, 64 targets, n=4096, k=48 (T 8192x96), "
-                               "device-resident", "batch": BATCH, "n": N, "k": K},
+        "config": {"workload": "batched FP64 Dual, 64 targets, n=4096, k=48 (T 8192x96), device-resident",
+                   "batch": BATCH, "n": N, "k": K},
+        "split_batched_solves_per_s": BATCH / (ms_split * 1e-3),
+        "split_stage_ms": stages_split,
+        "split_acceptance": acc_split,
+        "split_vs_fp64_rel_X": split_vs_fp64,
+        "fwd_bwd_ms_split_forward": fb[True],
         "plain_solves_per_s": BATCH / (plain_ms * 1e-3) if plain_ms else None,
         "single_call_solves_per_s": BATCH / (fused_ms * 1e-3),
         "pcie_inclusive_solves_per_s": BATCH / (pcie_ms * 1e-3),
@@ -237,7 +264,6 @@ def main():
         "bwd_ms": bwd_ms,
         "adjoint": {"cg_mean": adj_it, "all_converged": adj_ok},
         "stage_ms": stages,
-        "streaming_tflops": f_stream / ((gram_ms + rec_ms) * 1e-3) / 1e12,
         "dual_phase_ms": eigh_ms + newton_ms,
         "forward_bytes_per_step": bytes_fwd,
         "dual_flops_per_step": fd,
@@ -247,6 +273,7 @@ def main():
         "acceptance": acc,
         "roofline": dominant,
         "roofline_kernels": kernels,
+        "roofline_kernels_split": kernels_split,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

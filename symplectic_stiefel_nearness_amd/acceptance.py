@@ -7,7 +7,8 @@ projection of `baselines/geometry.py:22-39` and the thresholds of `reproduce.py:
     r_G   = ||Proj_Y(2 (Y - T))|| / (2 max(1, ||T||))
     delta = |objective - dual| / max(1, |objective|),  dual = ||T||^2 - ||L^-1 B^H||^2 - <omega, J>
 
-(||L^-1 B^H||^2 is evaluated as Re tr((I + i omega)^-1 B^H B), the same number.)
+(||L^-1 B^H||^2 is evaluated as Re tr((I + i omega)^-1 B^H B) = tr(P Cr) - tr(Q Ci), the same
+number, with the real inverse parts P, Q from the symmetric eigendecomposition of omega^T omega.)
 
 with omega the multiplier of the projection / 2 and L the Cholesky factor of I + i omega.
 This is a checker for reports and benchmarks (torch.linalg on FP64); it is not on the
@@ -64,18 +65,22 @@ def inspect(T, Y):
     gradient = value - jy @ multiplier
     stat = torch.linalg.vector_norm(gradient, dim=(-2, -1)) / (2 * scale)
     omega = multiplier / 2
-    kc = eye.to(torch.complex128) + 1j * omega.to(torch.complex128)
-    b = torch.complex(T[:, :n], T[:, n:])
-    factor, info = torch.linalg.cholesky_ex(kc)
-    # ||L^-1 B^H||_F^2 = Re tr((L L^H)^-1 B^H B): the same quantity through the m x m
-    # complex Gram, which keeps the device work at m x m (a batched n-column triangular
-    # solve needs more library workspace than hipBLAS provides at n = 4096)
-    gram = b.mH.contiguous() @ b
-    kinv = torch.cholesky_inverse(factor)
-    solved_sq = (kinv * gram.mT).sum((-2, -1)).real
+    # ||L^-1 B^H||_F^2 = Re tr((I + i omega)^-1 B^H B) = tr(P Cr) - tr(Q Ci) with
+    # (I + i omega)^-1 = P + i Q, P = (I + omega^2)^-1 = (I - omega^T omega)^-1, Q = -omega P,
+    # and B^H B = Cr + i Ci. I + i omega is positive definite exactly when every eigenvalue s
+    # of omega^T omega is below 1 (its Cholesky factor L exists); P comes from that
+    # eigendecomposition, so the check needs no complex factorisation.
+    s_vals, s_vecs = torch.linalg.eigh(_sym(omega.mT @ omega))
+    pd = (s_vals < 1.0).all(-1)
+    P = s_vecs @ (s_vecs.mT / (1.0 - s_vals).clamp_min(1e-300)[..., :, None])
+    Q = -omega @ P
+    A, D = T[:, :n], T[:, n:]
+    Cr = A.mT @ A + D.mT @ D
+    Ci = A.mT @ D - D.mT @ A
+    solved_sq = (P * Cr).sum((-2, -1)) - (Q.mT * Ci).sum((-2, -1))
     dual = (T * T).sum((-2, -1)) - solved_sq - (omega * jm).sum((-2, -1))
     delta = ((objective - dual) / objective.abs().clamp_min(1.0)).abs()
-    delta = torch.where(info == 0, delta, torch.full_like(delta, float("nan")))
+    delta = torch.where(pd, delta, torch.full_like(delta, float("nan")))
     ok = torch.isfinite(Y).flatten(1).all(1) & (values.min(-1).values > 0)
     passed = ok.clone()
     for key, tensor in (("feasibility", feas), ("stationarity", stat), ("delta", delta)):

@@ -4,13 +4,14 @@
 // Slicing (Ozaki-style, exact): a value x in a group that shares the power-of-two scale
 // 2^E (max |x| < 2^E) is written u = x 2^-E in (-1, 1) and cut as
 //     c_1 = rint(128 u),  u_1 = 128 u - c_1,  c_a = rint(256 u_{a-1}),  u_a = 256 u_{a-1} - c_a,
-// so u = sum_a c_a 2^(-7 - 8 (a - 1)) + remainder, |remainder| <= 2^(-8 S + 1) / 2, with every
-// |c_a| <= 128: each c_a is an integer of at most 8 significant bits, exact in BF16. The
-// product of slices a and b is an integer of magnitude <= 2^14 with weight
-// 2^(-14 - 8 (a + b - 2)); products of one order d = a + b are summed in their own FP32
-// accumulator, which stays an exact integer while (d - 1) * K * 2^14 < 2^24 (K = summed
-// length). Products of order d > S + 1 are dropped (below the truncation level). The
-// order sums are converted to FP64, weighted and added: the only rounding happens in FP64.
+// so u = 2^-7 sum_a s_a + remainder with slices s_a = c_a 2^(-8 (a - 1)), |c_a| <= 128 (at
+// most 8 significant bits: exact in BF16) and |remainder| <= 2^(-8 S - 1). A product of two
+// values is 2^-14 sum_{a,b} s_a s_b. The slice products of orders a + b = 2, 3 and 4 are
+// summed in FP32 accumulators of their own, which stay exact: order d's terms are integer
+// multiples of 2^(-8 (d - 2)) of magnitude <= 2^(14 - 8 (d - 2)), so the sum stays within
+// 24 bits while (#products per K index) * K * 2^14 <= 2^24. The products of orders
+// 5 .. S + 1 share a fourth accumulator whose FP32 rounding is 2^-24 of a sum 2^-24 below
+// the total. Orders above S + 1 are dropped. The sums are converted to FP64 and added there.
 //
 // MFMA 16x16x32 BF16 operand layout (gfx950), lane l = r + 16 g:
 //     A (16 x 32): lane holds A[r][8 g + j], j = 0..7
@@ -27,6 +28,7 @@ constexpr int SPLIT_S = 5;          // slices per value (39 bits below the group
 constexpr int SPLIT_THREADS = 512;
 constexpr int SPLIT_WAVES = SPLIT_THREADS / WAVE;
 constexpr int SG_ROWS = 32;         // rows per Gram stage (one MFMA K step)
+constexpr int NACC = 4;             // accumulators: orders 2, 3, 4, then 5 .. S + 1
 
 __device__ __forceinline__ f4 mfma_bf16(bf16x8 a, bf16x8 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -34,8 +36,7 @@ __device__ __forceinline__ f4 mfma_bf16(bf16x8 a, bf16x8 b, f4 c) {
 
 __device__ __forceinline__ f4 zero_f4() { f4 z = {0.f, 0.f, 0.f, 0.f}; return z; }
 
-// weight of the order-d sum (d = a + b >= 2): 2^(-14 - 8 (d - 2))
-__device__ __forceinline__ double order_weight(int d) { return ldexp(1.0, -14 - 8 * (d - 2)); }
+__host__ __device__ constexpr int acc_of(int a, int b) { return a + b <= 4 ? a + b - 2 : 3; }
 
 // exponent E with |v| < 2^E for every |v| <= vmax (0 for vmax == 0)
 __device__ __forceinline__ int scale_exponent(double vmax) {
@@ -44,23 +45,49 @@ __device__ __forceinline__ int scale_exponent(double vmax) {
     return vmax > 0.0 ? e : 0;
 }
 
+// Cuts 8 values (times `scale`) into S BF16 slices packed as MFMA operands. The slices
+// are exact in BF16, so the BF16 bits are the upper half of the FP32 bits (no rounding
+// step); two slices share one 32-bit word (v_perm_b32).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 template <int S>
-__device__ __forceinline__ void cut(double u, float (&c)[S]) {
-    double v = u * 128.0;
+__device__ __forceinline__ void cut8(const double (&v)[8], double scale, bf16x8 (&out)[S]) {
+    unsigned int bits[S][8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        double x = v[j] * scale * 128.0;
+        float w = 1.0f;
+#pragma unroll
+        for (int a = 0; a < S; ++a) {
+            const double q = rint(x);
+            bits[a][j] = __float_as_uint((float)q * w);    // c_a 2^(-8 a'), exact
+            x = (x - q) * 256.0;
+            w *= 0x1p-8f;
+        }
+    }
 #pragma unroll
     for (int a = 0; a < S; ++a) {
-        const double q = rint(v);
-        c[a] = (float)q;
-        v = (v - q) * 256.0;
+        u32x4 wv;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) wv[p] = __builtin_amdgcn_perm(bits[a][2 * p + 1], bits[a][2 * p], 0x07060302u);
+        out[a] = __builtin_bit_cast(bf16x8, wv);
     }
 }
 
-// rows of K summed exactly in one FP32 order accumulator whose largest order takes
-// C * S products per K index (C = 1, or 2 for the two chains of A^T A + D^T D)
-template <int S, int C>
-struct SplitExact {
-    static constexpr int K = ((1 << 24) / (C * S * (1 << 14))) / 32 * 32;   // S = 5: 192 / 96
-};
+// the accumulator sums as one FP64 value, times 2^-14 (smallest first)
+__device__ __forceinline__ double reassemble(const f4 (&acc)[NACC], int i) {
+    return ((((double)acc[3][i] + (double)acc[2][i]) + (double)acc[1][i]) + (double)acc[0][i]) * 0x1p-14;
+}
+
+// multiply-accumulate all slice products of order <= S + 1 into the three accumulators
+template <int S>
+__device__ __forceinline__ void mfma_products(const bf16x8 (&x)[S], const bf16x8 (&y)[S], f4 (&acc)[NACC]) {
+#pragma unroll
+    for (int a = 1; a <= S; ++a)
+#pragma unroll
+        for (int b = 1; b <= S + 1 - a; ++b)
+            acc[acc_of(a, b)] = mfma_bf16(x[a - 1], y[b - 1], acc[acc_of(a, b)]);
+}
 
 // ---------------------------------------------------------------- split Gram
 // One workgroup per (row chunk, tile group, target). Pass 1 takes the maximum of
@@ -69,9 +96,9 @@ struct SplitExact {
 // stages of 32 rows: every thread cuts 8 rows of one column into S slices and stores
 // them column-major in LDS, [slice][A|D][col][32 rows], so that an MFMA operand is one
 // 16-byte read. The tile group's waves own fixed output tiles (upper tiles of
-// A^T A + D^T D, all tiles of A^T D) with one FP32 accumulator per order; every
-// SplitExact<S>::K rows the orders are folded into FP64 accumulators. The FP64 partial is
-// written in the layout of the FP64 Gram so the same reduction kernel finishes it.
+// A^T A + D^T D, all tiles of A^T D). The order-4 accumulator of an A^T A + D^T D tile
+// takes 6 products per row, so every 160 rows the accumulators are folded into FP64.
+// The FP64 partial is written in the layout of the FP64 Gram, whose reduction finishes it.
 template <int NT, int G>
 struct SGShape {
     static constexpr int MP = NT * 16;
@@ -82,6 +109,7 @@ struct SGShape {
     static constexpr int UNITS = 2 * MP * (SG_ROWS / 8);        // (A|D, col, 8-row group)
     static constexpr int UPT = (UNITS + SPLIT_THREADS - 1) / SPLIT_THREADS;
     static constexpr int SLICE = 2 * MP * SG_ROWS;              // bf16 per slice
+    static constexpr int FOLD = 160 / SG_ROWS;                  // stages between FP64 folds
 };
 
 __device__ __forceinline__ void sg_tile(int t, int nt, int ncr, int& I, int& J, bool& is_cr) {
@@ -119,9 +147,23 @@ void split_gram_kernel(const double* __restrict__ T, long long sT_b, long long s
     {
         const int c = tid % 128, ro = tid / 128;
         double mx = 0.0;
-        if (c < m)
-            for (int row = row_begin + ro; row < row_end; row += 4)
-                mx = fmax(mx, fmax(fabs(Tb[(long long)row * sT_r + c]), fabs(Tb[(long long)(row + n) * sT_r + c])));
+        if (c < m) {
+            // 8 rows per iteration: 16 independent loads in flight, row offsets uniform
+            int row = row_begin + ro;
+            const double* pa = Tb + (long long)row * sT_r + c;
+            const long long dn = (long long)n * sT_r, st4 = 4 * sT_r;
+            for (; row + 28 < row_end; row += 32, pa += 8 * st4) {
+                double v[16];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    v[2 * u] = pa[u * st4];
+                    v[2 * u + 1] = pa[u * st4 + dn];
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u) mx = fmax(mx, fabs(v[u]));
+            }
+            for (; row < row_end; row += 4, pa += st4) mx = fmax(mx, fmax(fabs(pa[0]), fabs(pa[dn])));
+        }
         if (c < MP) colmax[ro][c] = mx;
     }
     __syncthreads();
@@ -139,43 +181,37 @@ void split_gram_kernel(const double* __restrict__ T, long long sT_b, long long s
         tok[s] = t < Sh::NTOT;
         sg_tile(tok[s] ? t : 0, NT, Sh::NCR, tI[s], tJ[s], tcr[s]);
     }
-    f4 acc[Sh::MAXT][S];
+    f4 acc[Sh::MAXT][NACC];
     d4 acc64[Sh::MAXT];
 #pragma unroll
     for (int s = 0; s < Sh::MAXT; ++s) {
         acc64[s] = zero4();
 #pragma unroll
-        for (int d = 0; d < S; ++d) acc[s][d] = zero_f4();
+        for (int d = 0; d < NACC; ++d) acc[s][d] = zero_f4();
     }
     auto fold = [&]() {
 #pragma unroll
         for (int s = 0; s < Sh::MAXT; ++s) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                double v = 0.0;
+            for (int i = 0; i < 4; ++i) acc64[s][i] += reassemble(acc[s], i);
 #pragma unroll
-                for (int d = S - 1; d >= 0; --d) v += (double)acc[s][d][i] * order_weight(d + 2);
-                acc64[s][i] += v;
-            }
-#pragma unroll
-            for (int d = 0; d < S; ++d) acc[s][d] = zero_f4();
+            for (int d = 0; d < NACC; ++d) acc[s][d] = zero_f4();
         }
     };
 
-    // staging unit u -> (half, column, 8-row group); lanes run along columns (coalesced)
+    // staging unit u -> (8-row group g8, column c, half h), g8 fastest: a wave's slice
+    // stores are 1 KB contiguous and its loads read 16 columns x 4 rows per instruction
     double pre[Sh::UPT][8];
     auto load = [&](int r0) {
 #pragma unroll
         for (int k = 0; k < Sh::UPT; ++k) {
             const int u = tid + k * SPLIT_THREADS;
-            const int c = u % MP, rest = u / MP, g8 = rest % 4, h = rest / 4;
+            const int g8 = u % 4, c = (u / 4) % MP, h = u / (4 * MP);
+            const int first = r0 + 8 * g8;
+            const int cnt = (u < Sh::UNITS && c < m) ? min(8, row_end - first) : 0;
+            const double* p = Tb + (long long)(first + h * n) * sT_r + c;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int row = r0 + 8 * g8 + j;
-                double v = 0.0;
-                if (u < Sh::UNITS && c < m && row < row_end) v = Tb[(long long)(row + h * n) * sT_r + c];
-                pre[k][j] = v;
-            }
+            for (int j = 0; j < 8; ++j) pre[k][j] = j < cnt ? p[j * sT_r] : 0.0;
         }
     };
     auto store = [&]() {
@@ -183,24 +219,21 @@ void split_gram_kernel(const double* __restrict__ T, long long sT_b, long long s
         for (int k = 0; k < Sh::UPT; ++k) {
             const int u = tid + k * SPLIT_THREADS;
             if (u >= Sh::UNITS) continue;
-            const int c = u % MP, rest = u / MP, g8 = rest % 4, h = rest / 4;
-            const double sc = colscale[c];
+            const int g8 = u % 4, c = (u / 4) % MP, h = u / (4 * MP);
             bf16x8 out[S];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                float cs[S];
-                cut<S>(pre[k][j] * sc, cs);
-#pragma unroll
-                for (int a = 0; a < S; ++a) out[a][j] = (__bf16)cs[a];
-            }
+            cut8<S>(pre[k], colscale[c], out);
 #pragma unroll
             for (int a = 0; a < S; ++a)
                 *reinterpret_cast<bf16x8*>(sl + a * Sh::SLICE + (h * MP + c) * SG_ROWS + 8 * g8) = out[a];
         }
     };
+    auto frag = [&](int col, bf16x8 (&x)[S]) {
+#pragma unroll
+        for (int a = 0; a < S; ++a)
+            x[a] = *reinterpret_cast<const bf16x8*>(sl + a * Sh::SLICE + (col + r) * SG_ROWS + 8 * g);
+    };
 
     const int nstages = (row_end - row_begin + SG_ROWS - 1) / SG_ROWS;
-    constexpr int FOLD = SplitExact<S, 2>::K / SG_ROWS;
     if (nstages > 0) load(row_begin);
     for (int st = 0; st < nstages; ++st) {
         store();
@@ -210,39 +243,17 @@ void split_gram_kernel(const double* __restrict__ T, long long sT_b, long long s
         for (int s = 0; s < Sh::MAXT; ++s) {
             if (!tok[s]) continue;
             const int I = tI[s] * 16, J = tJ[s] * 16;
-            // operand fragments of this tile: A columns I (and J for Cr), D columns J (and I)
-            bf16x8 aI[S], xJ[S];
-#pragma unroll
-            for (int a = 0; a < S; ++a) {
-                aI[a] = *reinterpret_cast<const bf16x8*>(sl + a * Sh::SLICE + (I + r) * SG_ROWS + 8 * g);
-                xJ[a] = *reinterpret_cast<const bf16x8*>(sl + a * Sh::SLICE + ((tcr[s] ? 0 : MP) + J + r) * SG_ROWS + 8 * g);
-            }
-#pragma unroll
-            for (int d = 2; d <= S + 1; ++d)
-#pragma unroll
-                for (int a = 1; a < d; ++a) {
-                    const int bb = d - a;
-                    if (a > S || bb > S) continue;
-                    acc[s][d - 2] = mfma_bf16(aI[a - 1], xJ[bb - 1], acc[s][d - 2]);
-                }
+            bf16x8 x[S], y[S];
+            frag(I, x);                               // A columns I
+            frag((tcr[s] ? 0 : MP) + J, y);           // A columns J (Cr) or D columns J (A^T D)
+            mfma_products<S>(x, y, acc[s]);
             if (tcr[s]) {
-                bf16x8 dI[S], dJ[S];
-#pragma unroll
-                for (int a = 0; a < S; ++a) {
-                    dI[a] = *reinterpret_cast<const bf16x8*>(sl + a * Sh::SLICE + (MP + I + r) * SG_ROWS + 8 * g);
-                    dJ[a] = *reinterpret_cast<const bf16x8*>(sl + a * Sh::SLICE + (MP + J + r) * SG_ROWS + 8 * g);
-                }
-#pragma unroll
-                for (int d = 2; d <= S + 1; ++d)
-#pragma unroll
-                    for (int a = 1; a < d; ++a) {
-                        const int bb = d - a;
-                        if (a > S || bb > S) continue;
-                        acc[s][d - 2] = mfma_bf16(dI[a - 1], dJ[bb - 1], acc[s][d - 2]);
-                    }
+                frag(MP + I, x);                      // D columns I and J
+                frag(MP + J, y);
+                mfma_products<S>(x, y, acc[s]);
             }
         }
-        if ((st + 1) % FOLD == 0 || st + 1 == nstages) fold();
+        if ((st + 1) % Sh::FOLD == 0 || st + 1 == nstages) fold();
         __syncthreads();
     }
 
@@ -263,9 +274,10 @@ void split_gram_kernel(const double* __restrict__ T, long long sT_b, long long s
 
 // ---------------------------------------------------------------- split recovery
 // X = W M with W = [A | D] (n x 2 mp, row r of A next to row r of D) and
-// M = [[P^T, -Q^T], [Q^T, P^T]] (2 mp x 2 mp). Each row of W shares one exponent (row
-// maximum, taken in registers), each column of M one exponent (prepared with M's slices by
-// split_recovery_fragments). K = 2 mp is summed in segments of at most SplitExact<S>::K.
+// M = [[P^T, -Q^T], [Q^T, P^T]] (2 mp x 2 mp). Each row of W shares one exponent (its row
+// maximum), each column of M one exponent (prepared with M's slices by
+// split_recovery_fragments). The order-4 sum takes 3 products per K index and
+// K = 2 mp <= 256, so one FP32 pass is exact.
 //
 // Fragments: frag[b] = [S][KS][2 NT][64 lanes] bf16x8 (B operands of every (k step,
 // column tile)), then colpow[b] = [2 mp] doubles 2^(F_c).
@@ -312,30 +324,33 @@ __global__ void split_recovery_fragments(const double* __restrict__ P, const dou
     __syncthreads();
     for (int u = tid; u < KS * 64; u += blockDim.x) {
         const int ks = u / 64, lane = u % 64, r = lane & 15, g = lane >> 4;
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = Mv(32 * ks + 8 * g + j, 16 * J + r);
         bf16x8 out[S];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            float cs[S];
-            cut<S>(Mv(32 * ks + 8 * g + j, 16 * J + r) * csc[r], cs);
-#pragma unroll
-            for (int a = 0; a < S; ++a) out[a][j] = (__bf16)cs[a];
-        }
+        cut8<S>(v, csc[r], out);
 #pragma unroll
         for (int a = 0; a < S; ++a)
             *reinterpret_cast<bf16x8*>(fb + (((long long)(a * KS + ks) * NJ + J) * 64 + lane) * 8) = out[a];
     }
 }
 
-// Workgroup: 8 waves = RB row blocks of 16 rows x CG column groups (NT <= 6: 4 x 2,
-// else 2 x 4); a wave owns up to ceil(2 NT / CG) column tiles of its 16 rows.
+// Workgroup of 8 waves over ROWS rows: the row exponents are taken cooperatively (TPR
+// threads per row), then for every K step the workgroup cuts its ROWS x 32 values into
+// slices in LDS once (double-buffered, one barrier per step) and every wave multiplies two
+// row blocks of 16 by its TPW column tiles, so each B fragment it loads serves 32 rows.
+// Waves: RBP row-block pairs x CG column groups (NT <= 4: 4 x 2, else 2 x 4).
 template <int NT>
 struct SRShape {
     static constexpr int MP = NT * 16;
-    static constexpr int CG = NT <= 6 ? 2 : 4;
-    static constexpr int RB = SPLIT_WAVES / CG;
-    static constexpr int ROWS = 16 * RB;
+    static constexpr int CG = NT <= 4 ? 2 : 4;
+    static constexpr int RBP = SPLIT_WAVES / CG;
+    static constexpr int ROWS = 32 * RBP;
     static constexpr int TPW = (2 * NT + CG - 1) / CG;   // column tiles per wave
     static constexpr int KS = 2 * MP / 32;               // K steps
+    static constexpr int TPR = SPLIT_THREADS / ROWS;     // threads per row (exponents)
+    static constexpr int SEGLEN = 2 * MP / TPR;          // values per thread (exponents)
+    static constexpr int UNITS = ROWS * 4;               // (row, 8-value group) per K step
 };
 
 template <int NT, int S>
@@ -345,121 +360,109 @@ void split_recover_kernel(const double* __restrict__ T, long long sT_b, long lon
                           double* __restrict__ X, long long sX_b, long long sX_r)
 {
     using Sh = SRShape<NT>;
-    constexpr int MP = Sh::MP, KS = Sh::KS, NJ = 2 * NT;
-    constexpr int SEG = SplitExact<S, 1>::K / 32;          // K steps per exact segment
-    const int b = blockIdx.y;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    constexpr int MP = Sh::MP, KS = Sh::KS, NJ = 2 * NT, ROWS = Sh::ROWS;
+    __shared__ __align__(16) __bf16 as_lds[2][S][ROWS * 32];
+    __shared__ double rscale[ROWS];
+    __shared__ int rexp[ROWS];
+    const int b = blockIdx.y, tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
-    const int rb = wave / Sh::CG, cg = wave % Sh::CG;
-    const int row = blockIdx.x * Sh::ROWS + rb * 16 + r;
-    const bool row_ok = row < n;
+    const int row0 = blockIdx.x * ROWS;
     const double* Tb = T + (long long)b * sT_b;
     const __bf16* fb = reinterpret_cast<const __bf16*>(frag + (long long)b * split_frag_doubles<S>(MP));
     const double* colpow = frag + (long long)b * split_frag_doubles<S>(MP) + (long long)S * MP * MP;
 
-    // the 8 values of W[row][32 ks + 8 g .. + 7] (one part: A if k < MP else D)
-    auto wvals = [&](int ks, double (&v)[8]) {
-        const int k0 = 32 * ks + 8 * g;
-        const int part = k0 / MP, c0 = k0 - part * MP;
-        const double* src = Tb + (long long)(row + part * n) * sT_r + c0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = (row_ok && c0 + j < m) ? src[j] : 0.0;
+    // the values W[row][k0 .. k0 + len) (one part: k0 and the run stay in A or in D), zero
+    // outside the target, through one base pointer
+    auto wrun = [&](int row, int k0, int len, double* v) {
+        const int part = k0 >= MP ? 1 : 0, c0 = k0 - part * MP;
+        const int cnt = row < n ? min(len, m - c0) : 0;
+        const double* p = Tb + (long long)(row + part * n) * sT_r + c0;
+        for (int j = 0; j < len; ++j) v[j] = j < cnt ? p[j] : 0.0;
     };
-    // row maximum: this lane's k values, then the 4 lanes of the row (lanes r + 16 g)
-    double mx = 0.0;
-#pragma unroll 1
-    for (int ks = 0; ks < KS; ++ks) {
-        double v[8];
-        wvals(ks, v);
+    // row exponents: TPR consecutive threads share a row, each takes SEGLEN values
+    {
+        const int lr = tid / Sh::TPR, seg = tid % Sh::TPR;
+        double mx = 0.0;
+        double v[Sh::SEGLEN];
+        wrun(row0 + lr, seg * Sh::SEGLEN, Sh::SEGLEN, v);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) mx = fmax(mx, fabs(v[j]));
+        for (int j = 0; j < Sh::SEGLEN; ++j) mx = fmax(mx, fabs(v[j]));
+#pragma unroll
+        for (int o = 1; o < Sh::TPR; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o));
+        if (seg == 0) {
+            const int e = scale_exponent(mx);
+            rexp[lr] = e;
+            rscale[lr] = ldexp(1.0, -e);
+        }
     }
-    mx = fmax(mx, __shfl_xor(mx, 16));
-    mx = fmax(mx, __shfl_xor(mx, 32));
-    const int erow = scale_exponent(mx);
-    const double rsc = ldexp(1.0, -erow);
+    __syncthreads();
 
-    // K <= one exact segment (mp <= 96): the order sums go straight to the output
-    constexpr bool ONESEG = KS <= SEG;
-    f4 acc[Sh::TPW][S];
-    d4 acc64[ONESEG ? 1 : Sh::TPW];
+    const int rbp = wave / Sh::CG, cg = wave % Sh::CG;
+    f4 acc[2][Sh::TPW][NACC];
 #pragma unroll
-    for (int t = 0; t < (ONESEG ? 1 : Sh::TPW); ++t) acc64[t] = zero4();
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int t = 0; t < Sh::TPW; ++t) {
+        for (int t = 0; t < Sh::TPW; ++t)
 #pragma unroll
-        for (int d = 0; d < S; ++d) acc[t][d] = zero_f4();
-    }
+            for (int d = 0; d < NACC; ++d) acc[h][t][d] = zero_f4();
+
+    // cutting unit: (row, 8 consecutive k of the step)
+    auto cut_step = [&](int ks, int buf) {
+        for (int u = tid; u < Sh::UNITS; u += SPLIT_THREADS) {
+            const int lr = u / 4, g8 = u % 4;
+            double v[8];
+            wrun(row0 + lr, 32 * ks + 8 * g8, 8, v);
+            bf16x8 out[S];
+            cut8<S>(v, rscale[lr], out);
+#pragma unroll
+            for (int a = 0; a < S; ++a)
+                *reinterpret_cast<bf16x8*>(&as_lds[buf][a][lr * 32 + 8 * g8]) = out[a];
+        }
+    };
+
 #pragma unroll 1
     for (int ks = 0; ks < KS; ++ks) {
-        double v[8];
-        wvals(ks, v);
-        bf16x8 as[S];
+        const int buf = ks & 1;
+        cut_step(ks, buf);
+        __syncthreads();          // slices of step ks ready; step ks - 1's buffer no longer read
+        bf16x8 x0[S], x1[S];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            float cs[S];
-            cut<S>(v[j] * rsc, cs);
-#pragma unroll
-            for (int a = 0; a < S; ++a) as[a][j] = (__bf16)cs[a];
+        for (int a = 0; a < S; ++a) {
+            x0[a] = *reinterpret_cast<const bf16x8*>(&as_lds[buf][a][(rbp * 32 + r) * 32 + 8 * g]);
+            x1[a] = *reinterpret_cast<const bf16x8*>(&as_lds[buf][a][(rbp * 32 + 16 + r) * 32 + 8 * g]);
         }
 #pragma unroll
         for (int t = 0; t < Sh::TPW; ++t) {
             const int J = cg * Sh::TPW + t;
             if (J >= NJ) continue;
-            bf16x8 bs[S];
+            bf16x8 y[S];
 #pragma unroll
             for (int a = 0; a < S; ++a)
-                bs[a] = *reinterpret_cast<const bf16x8*>(fb + (((long long)(a * KS + ks) * NJ + J) * 64 + lane) * 8);
-#pragma unroll
-            for (int d = 2; d <= S + 1; ++d)
-#pragma unroll
-                for (int a = 1; a < d; ++a) {
-                    const int bb = d - a;
-                    if (a > S || bb > S) continue;
-                    acc[t][d - 2] = mfma_bf16(as[a - 1], bs[bb - 1], acc[t][d - 2]);
-                }
-        }
-        if (!ONESEG && ((ks + 1) % SEG == 0 || ks + 1 == KS)) {
-#pragma unroll
-            for (int t = 0; t < (ONESEG ? 1 : Sh::TPW); ++t) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    double s_ = 0.0;
-#pragma unroll
-                    for (int d = S - 1; d >= 0; --d) s_ += (double)acc[t][d][i] * order_weight(d + 2);
-                    acc64[t][i] += s_;
-                }
-#pragma unroll
-                for (int d = 0; d < S; ++d) acc[t][d] = zero_f4();
-            }
+                y[a] = *reinterpret_cast<const bf16x8*>(fb + (((long long)(a * KS + ks) * NJ + J) * 64 + lane) * 8);
+            mfma_products<S>(x0, y, acc[0][t]);
+            mfma_products<S>(x1, y, acc[1][t]);
         }
     }
-    // lane holds rows 4 g + i of the block, column 16 J + r: exponents of those rows
-    int erows[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) erows[i] = __shfl(erow, 4 * g + i);
     double* Xb = X + (long long)b * sX_b;
-    const int row0 = blockIdx.x * Sh::ROWS + rb * 16;
 #pragma unroll
-    for (int t = 0; t < Sh::TPW; ++t) {
-        if (cg * Sh::TPW + t >= NJ) continue;
-        const int c = (cg * Sh::TPW + t) * 16 + r;
-        const bool bottom = c >= MP;
-        const int cc = bottom ? c - MP : c;
-        const double cp = colpow[c];
+    for (int h = 0; h < 2; ++h) {
+        const int lb = rbp * 32 + 16 * h;                   // local first row of the block
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int rr = row0 + 4 * g + i;
-            double v;
-            if (ONESEG) {
-                v = 0.0;
+        for (int t = 0; t < Sh::TPW; ++t) {
+            const int J = cg * Sh::TPW + t;
+            if (J >= NJ) continue;
+            const int c = J * 16 + r;
+            const bool bottom = c >= MP;
+            const int cc = bottom ? c - MP : c;
+            const double cp = colpow[c];
 #pragma unroll
-                for (int d = S - 1; d >= 0; --d) v += (double)acc[t][d][i] * order_weight(d + 2);
-            } else {
-                v = acc64[ONESEG ? 0 : t][i];
+            for (int i = 0; i < 4; ++i) {
+                const int lr = lb + 4 * g + i, rr = row0 + lr;
+                const double v = reassemble(acc[h][t], i);
+                if (rr < n && cc < m)
+                    Xb[(long long)(rr + (bottom ? n : 0)) * sX_r + cc] = ldexp(v, rexp[lr]) * cp;
             }
-            if (rr < n && cc < m)
-                Xb[(long long)(rr + (bottom ? n : 0)) * sX_r + cc] = ldexp(v, erows[i]) * cp;
         }
     }
 }
