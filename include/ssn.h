@@ -127,6 +127,32 @@ int ssn_backward(void* stream, const double* T, long long sT_b, long long sT_r,
                  const double* G, long long sG_b, long long sG_r, int batch, int n, int m,
                  double* dT, long long sD_b, long long sD_r, int* counters, void* ws, size_t ws_bytes);
 
+/* ---- uneven batches: targets of mixed (n, m) in one launch per stage ---------------------
+ * No reference counterpart (the reference solves one shape per call, dual_recommended_hvp.py:
+ * 231-242). Use: bytes = ssn_uneven_workspace_bytes(t, count); allocate ws; plan_bytes =
+ * ssn_uneven_plan_bytes(t, count); ssn_uneven_plan(t, count, ws, bytes, plan, plan_bytes) fills the
+ * HOST buffer `plan` (device pointers in it point into ws); copy `plan` to the first plan_bytes
+ * of ws; then ssn_solve_uneven(stream, plan, ws, bytes) launches the stages (Gram, eigensolver,
+ * Newton, recovery), each one launch over every target, largest work first. The same plan and
+ * ws may be reused (and the launches captured in a graph) while the targets' shapes and
+ * pointers stay the same. gram_chunk_rows (multiple of 16, 0 = 512) and rec_rows (multiple of
+ * 64, 0 = 64) are the tile shapes an autotuner may choose per target. */
+typedef struct ssn_target {
+    const double* T;   long long sT_r;     /* target [2n][m], row stride in elements        */
+    double* X;         long long sX_r;     /* output [2n][m]                                */
+    double* omega;     long long ldo;      /* output [m][m]                                 */
+    long long* iters;  int* status;        /* 1 element each                                */
+    int* counters;                         /* 12 ints (as ssn_dual) or NULL                 */
+    int n, m;
+    int gram_chunk_rows, rec_rows;
+} ssn_target;
+
+size_t ssn_uneven_workspace_bytes(const ssn_target* targets, int count);
+size_t ssn_uneven_plan_bytes(const ssn_target* targets, int count);
+int ssn_uneven_plan(const ssn_target* targets, int count, void* ws, size_t ws_bytes, void* plan,
+                    size_t plan_bytes);
+int ssn_solve_uneven(void* stream, const void* plan, void* ws, size_t ws_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,5 +13,8 @@ from .dual import (initialize_kernels, solve, solve_device, project, backward_de
                    dual_batch, dual_batch_staged, recover_matmul, FEAS_TOL, MAX_NEWTON, MAX_CG,
                    MAX_BACKTRACK)
 
+from .uneven import UnevenSolver, solve_uneven
+from .autotune import autotune
+
 __all__ = ["initialize_kernels", "solve", "solve_device", "project", "backward_device", "gram_chunks",
-           "dual_batch", "dual_batch_staged", "recover_matmul"]
+           "dual_batch", "dual_batch_staged", "recover_matmul", "UnevenSolver", "solve_uneven", "autotune"]

@@ -24,6 +24,19 @@ _I = ctypes.c_int
 _L = ctypes.c_longlong
 _S = ctypes.c_size_t
 
+class SsnTarget(ctypes.Structure):
+    """Mirror of `ssn_target` (include/ssn.h)."""
+    _fields_ = [("T", ctypes.c_void_p), ("sT_r", ctypes.c_longlong),
+                ("X", ctypes.c_void_p), ("sX_r", ctypes.c_longlong),
+                ("omega", ctypes.c_void_p), ("ldo", ctypes.c_longlong),
+                ("iters", ctypes.c_void_p), ("status", ctypes.c_void_p),
+                ("counters", ctypes.c_void_p),
+                ("n", ctypes.c_int), ("m", ctypes.c_int),
+                ("gram_chunk_rows", ctypes.c_int), ("rec_rows", ctypes.c_int)]
+
+
+_TP = ctypes.POINTER(SsnTarget)
+
 # name -> (restype, argtypes); must match include/ssn.h exactly
 SIGNATURES = {
     "ssn_init": (_I, []),
@@ -42,6 +55,10 @@ SIGNATURES = {
     "ssn_gram_split": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _P, _L, _I, _P, _S]),
     "ssn_recover_split": (_I, [_P, _P, _L, _L, _P, _P, _I, _I, _I, _P, _L, _L, _P, _S]),
     "ssn_solve_split": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _L, _I, _P, _P, _P, _P, _S]),
+    "ssn_uneven_workspace_bytes": (_S, [_TP, _I]),
+    "ssn_uneven_plan_bytes": (_S, [_TP, _I]),
+    "ssn_uneven_plan": (_I, [_TP, _I, _P, _S, _P, _S]),
+    "ssn_solve_uneven": (_I, [_P, _P, _P, _S]),
 }
 
 _lib = None

@@ -4,7 +4,11 @@
 // scratch comes in through the `ws` argument, sized by ssn_workspace_bytes().
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <algorithm>
+#include <vector>
 #include "common.h"
+#include "uneven.h"
+#include "../../include/ssn.h"
 
 namespace ssn {
 int gram_chunk_count(int batch, int n);
@@ -30,6 +34,11 @@ hipError_t launch_split_gram(hipStream_t, const double*, long long, long long, i
 long long split_fragment_doubles(int m);
 hipError_t launch_split_recover(hipStream_t, const double*, long long, long long, const double*, const double*,
                                 double*, int, int, int, double*, long long, long long);
+hipError_t launch_gram_uneven(hipStream_t, const UTarget*, const UItem*, const int*, int);
+hipError_t launch_dual_uneven(hipStream_t, const UTarget*, int, size_t, size_t);
+hipError_t launch_recover_uneven(hipStream_t, const UTarget*, const UItem*, int, int, int);
+void eigh_variant(int m, size_t lds_limit, int* small, int* vlds, size_t* lds_bytes);
+size_t dual_lds_bytes(int m);
 }  // namespace ssn
 
 using namespace ssn;
@@ -302,6 +311,194 @@ int ssn_backward(void* stream, const double* T, long long sT_b, long long sT_r,
     return hip_status(launch_backward_stream(s, T, sT_b, sT_r, G, sG_b, sG_r, dual + SLOT_KR * mm,
                                              dual + SLOT_KI * mm, td, w + L.pf, w + L.qf, w + L.frag,
                                              batch, n, m, dT, sD_b, sD_r));
+}
+
+// ---------------------------------------------------------------- uneven batches
+namespace {
+
+constexpr size_t UNEVEN_ALIGN = 256;
+constexpr int UNEVEN_DEFAULT_CHUNK = 512;
+constexpr size_t GFX950_LDS_BYTES = 163840;     // planning default when no device is current
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+bool target_ok(const ssn_target& t) {
+    return t.T && t.X && t.omega && t.iters && t.status && t.n >= 1 && t.m >= 2 && (t.m % 2) == 0 &&
+           t.m <= 128 && t.ldo >= t.m && t.sT_r >= t.m && t.sX_r >= t.m &&
+           t.gram_chunk_rows >= 0 && t.rec_rows >= 0;
+}
+
+int chunk_rows_of(const ssn_target& t) {
+    const int c = t.gram_chunk_rows > 0 ? t.gram_chunk_rows : UNEVEN_DEFAULT_CHUNK;
+    return (c + 15) / 16 * 16;
+}
+int rec_rows_of(const ssn_target& t) {
+    const int r = t.rec_rows > 0 ? t.rec_rows : 64;
+    return (r + 63) / 64 * 64;
+}
+
+// Sizes of one target's workspace regions (doubles), in layout order.
+struct UTargetSizes {
+    size_t partial, cr, ci, dual, pf, qf, frag;
+    size_t total() const {
+        return align_up(partial * 8, UNEVEN_ALIGN) + align_up(cr * 8, UNEVEN_ALIGN) +
+               align_up(ci * 8, UNEVEN_ALIGN) + align_up(dual * 8, UNEVEN_ALIGN) +
+               align_up(pf * 8, UNEVEN_ALIGN) + align_up(qf * 8, UNEVEN_ALIGN) + align_up(frag * 8, UNEVEN_ALIGN);
+    }
+};
+
+UTargetSizes sizes_of(const ssn_target& t) {
+    const size_t mp = (size_t)pad16(t.m);
+    const size_t chunks = (size_t)((t.n + chunk_rows_of(t) - 1) / chunk_rows_of(t));
+    UTargetSizes z;
+    z.partial = chunks * 2 * mp * mp;
+    z.cr = z.ci = (size_t)t.m * t.m;
+    z.dual = (size_t)workspace_target_doubles(t.m);
+    z.pf = z.qf = mp * mp;
+    z.frag = 4 * mp * mp;
+    return z;
+}
+
+struct UCounts { int gram_items, rec_items; };
+
+UCounts counts_of(const ssn_target* t, int count) {
+    UCounts c{0, 0};
+    for (int i = 0; i < count; ++i) {
+        c.gram_items += (t[i].n + chunk_rows_of(t[i]) - 1) / chunk_rows_of(t[i]);
+        c.rec_items += (t[i].n + rec_rows_of(t[i]) - 1) / rec_rows_of(t[i]);
+    }
+    return c;
+}
+
+size_t plan_bytes_of(const ssn_target* t, int count) {
+    const UCounts c = counts_of(t, count);
+    return align_up(sizeof(UPlanHeader), UNEVEN_ALIGN) + align_up((size_t)count * sizeof(UTarget), UNEVEN_ALIGN) +
+           align_up((size_t)c.gram_items * sizeof(UItem), UNEVEN_ALIGN) +
+           align_up((size_t)c.rec_items * sizeof(UItem), UNEVEN_ALIGN);
+}
+
+bool targets_ok(const ssn_target* t, int count) {
+    if (!t || count <= 0) return false;
+    for (int i = 0; i < count; ++i)
+        if (!target_ok(t[i])) return false;
+    return true;
+}
+
+}  // namespace
+
+size_t ssn_uneven_plan_bytes(const ssn_target* targets, int count) {
+    if (!targets_ok(targets, count)) return 0;
+    return plan_bytes_of(targets, count);
+}
+
+size_t ssn_uneven_workspace_bytes(const ssn_target* targets, int count) {
+    if (!targets_ok(targets, count)) return 0;
+    size_t total = align_up(plan_bytes_of(targets, count), UNEVEN_ALIGN);
+    for (int i = 0; i < count; ++i) total += sizes_of(targets[i]).total();
+    return total;
+}
+
+int ssn_uneven_plan(const ssn_target* targets, int count, void* ws, size_t ws_bytes, void* plan, size_t plan_bytes)
+{
+    if (!targets_ok(targets, count) || !ws || !plan) return SSN_ERR_ARGS;
+    const size_t need_plan = plan_bytes_of(targets, count);
+    const size_t need_ws = ssn_uneven_workspace_bytes(targets, count);
+    if (plan_bytes < need_plan) return SSN_ERR_ARGS;
+    if (ws_bytes < need_ws) return SSN_ERR_WORKSPACE;
+    size_t lds_limit = GFX950_LDS_BYTES;
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+        lds_limit = (size_t)v;
+
+    char* pb = (char*)plan;
+    UPlanHeader* h = (UPlanHeader*)pb;
+    const UCounts c = counts_of(targets, count);
+    h->count = count; h->gram_items = c.gram_items; h->rec_items = c.rec_items;
+    h->targets_off = (long long)align_up(sizeof(UPlanHeader), UNEVEN_ALIGN);
+    h->gram_off = h->targets_off + (long long)align_up((size_t)count * sizeof(UTarget), UNEVEN_ALIGN);
+    h->rec_off = h->gram_off + (long long)align_up((size_t)c.gram_items * sizeof(UItem), UNEVEN_ALIGN);
+    h->plan_bytes = (long long)need_plan;
+    h->ws_bytes = (long long)need_ws;
+    UTarget* tg = (UTarget*)(pb + h->targets_off);
+    UItem* gi = (UItem*)(pb + h->gram_off);
+    UItem* ri = (UItem*)(pb + h->rec_off);
+
+    char* w = (char*)ws + align_up(need_plan, UNEVEN_ALIGN);
+    size_t eigh_lds = 0, dual_lds = 0;
+    int max_mp = 0;
+    std::vector<std::pair<long long, UItem>> gw, rw;
+    for (int i = 0; i < count; ++i) {
+        const ssn_target& t = targets[i];
+        const UTargetSizes z = sizes_of(t);
+        UTarget& u = tg[i];
+        u.T = t.T; u.sT_r = t.sT_r; u.X = t.X; u.sX_r = t.sX_r; u.omega = t.omega; u.ldo = t.ldo;
+        u.iters = t.iters; u.status = t.status; u.counters = t.counters;
+        u.n = t.n; u.m = t.m; u.mp = pad16(t.m);
+        u.chunk_rows = chunk_rows_of(t); u.rec_rows = rec_rows_of(t);
+        u.chunks = (t.n + u.chunk_rows - 1) / u.chunk_rows;
+        auto take = [&](size_t doubles) { double* p = (double*)w; w += align_up(doubles * 8, UNEVEN_ALIGN); return p; };
+        u.partial = take(z.partial); u.Cr = take(z.cr); u.Ci = take(z.ci); u.dual = take(z.dual);
+        u.Pf = take(z.pf); u.Qf = take(z.qf); u.frag = take(z.frag);
+        size_t el = 0;
+        eigh_variant(t.m, lds_limit, &u.eigh_small, &u.eigh_vlds, &el);
+        const size_t dl = dual_lds_bytes(t.m);
+        if (el > lds_limit || dl > lds_limit) return SSN_ERR_UNSUPPORTED;
+        eigh_lds = std::max(eigh_lds, el);
+        dual_lds = std::max(dual_lds, dl);
+        max_mp = std::max(max_mp, u.mp);
+        const long long mp2 = (long long)u.mp * u.mp;
+        for (int k = 0; k < u.chunks; ++k) {
+            const long long rows = std::min(u.chunk_rows, t.n - k * u.chunk_rows);
+            gw.push_back({rows * mp2, UItem{i, k}});
+        }
+        for (int r0 = 0; r0 < t.n; r0 += u.rec_rows) {
+            const long long rows = std::min(u.rec_rows, t.n - r0);
+            rw.push_back({rows * mp2, UItem{i, r0}});
+        }
+    }
+    // largest work first (ties: target, then index), so the persistent grids end together;
+    // the Gram items are grouped by order first (one launch per order)
+    auto by_cost = [](const std::pair<long long, UItem>& a, const std::pair<long long, UItem>& b) {
+        if (a.first != b.first) return a.first > b.first;
+        if (a.second.target != b.second.target) return a.second.target < b.second.target;
+        return a.second.index < b.second.index;
+    };
+    auto by_order_cost = [&](const std::pair<long long, UItem>& a, const std::pair<long long, UItem>& b) {
+        const int oa = tg[a.second.target].mp, ob = tg[b.second.target].mp;
+        if (oa != ob) return oa < ob;
+        return by_cost(a, b);
+    };
+    std::sort(gw.begin(), gw.end(), by_order_cost);
+    std::sort(rw.begin(), rw.end(), by_cost);
+    for (int k = 0; k <= 8; ++k) h->gram_begin[k] = 0;
+    for (size_t k = 0; k < gw.size(); ++k) h->gram_begin[tg[gw[k].second.target].mp / 16]++;
+    for (int k = 1; k <= 8; ++k) h->gram_begin[k] += h->gram_begin[k - 1];
+    for (size_t k = 0; k < gw.size(); ++k) gi[k] = gw[k].second;
+    for (size_t k = 0; k < rw.size(); ++k) ri[k] = rw[k].second;
+    h->eigh_lds = (long long)eigh_lds;
+    h->dual_lds = (long long)dual_lds;
+    h->max_mp = max_mp;
+    return SSN_OK;
+}
+
+int ssn_solve_uneven(void* stream, const void* plan, void* ws, size_t ws_bytes)
+{
+    if (!plan || !ws) return SSN_ERR_ARGS;
+    const UPlanHeader* h = (const UPlanHeader*)plan;
+    if (h->count <= 0 || ws_bytes < (size_t)h->ws_bytes) return SSN_ERR_WORKSPACE;
+    int rc = current_device_ready(nullptr);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const char* dp = (const char*)ws;           // the device copy of the plan
+    const UTarget* tg = (const UTarget*)(dp + h->targets_off);
+    const UItem* gi = (const UItem*)(dp + h->gram_off);
+    const UItem* ri = (const UItem*)(dp + h->rec_off);
+    rc = hip_status(launch_gram_uneven(s, tg, gi, h->gram_begin, h->count));
+    if (rc) return rc;
+    rc = hip_status(launch_dual_uneven(s, tg, h->count, (size_t)h->eigh_lds, (size_t)h->dual_lds));
+    if (rc) return rc;
+    return hip_status(launch_recover_uneven(s, tg, ri, h->rec_items, h->count, h->max_mp));
 }
 
 }  // extern "C"

@@ -25,6 +25,7 @@
 // reference's preconditioner skew(R / den) equals R / den bit for bit; it is computed
 // that way.
 #include "common.h"
+#include "uneven.h"
 
 // element-wise updates mirror the reference's explicitly rounded (non-contracted) ops
 #pragma clang fp contract(off)
@@ -775,15 +776,15 @@ __host__ __device__ __forceinline__ long long target_doubles(int mp) {
     return (long long)S_COUNT * mp * mp + mp + 16;
 }
 
+// Body of target b's eigensolver workgroup (ET threads, dynamic LDS eigh_lds_doubles).
 template <int JB, bool V_IN_LDS>
-__global__ __launch_bounds__(ET)
-void eigh_kernel(EighArgs a)
+__device__ __forceinline__ void eigh_body(const EighArgs& a, int b)
 {
     extern __shared__ double lds[];
     Ctx c;
     c.tid = threadIdx.x; c.wave = __builtin_amdgcn_readfirstlane(c.tid >> 6); c.lane = c.tid & 63;
     c.r = c.lane & 15; c.q = c.lane >> 4; c.m = a.m; c.red = lds; c.phase = 0;
-    const int b = blockIdx.x, m = a.m, mp = a.mp;
+    const int m = a.m, mp = a.mp;
     const unsigned long long t0 = wall_clock64();
     double* ws = a.ws + (long long)b * target_doubles(mp);
     double* Vout = ws + (long long)S_V * mp * mp;
@@ -857,10 +858,17 @@ void eigh_kernel(EighArgs a)
     }
 }
 
+template <int JB, bool V_IN_LDS>
+__global__ __launch_bounds__(ET)
+void eigh_kernel(EighArgs a)
+{
+    eigh_body<JB, V_IN_LDS>(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------- the engine
+// Body of target b's workgroup (DT threads, dynamic LDS Cfg<NT>::LDS_END doubles).
 template <int NT>
-__global__ __launch_bounds__(DT)
-void dual_kernel(DualArgs a)
+__device__ __forceinline__ void dual_body(const DualArgs& a, int b)
 {
     using Cf = Cfg<NT>;
     constexpr int MP = Cf::MP;
@@ -872,7 +880,6 @@ void dual_kernel(DualArgs a)
     c.m = a.m;
     c.red = lds;
     c.phase = 0;
-    const int b = blockIdx.x;
     const int m = c.m;
     const unsigned long long t_start = wall_clock64();
     unsigned long long t_cg = 0, t_ls = 0, t_setup = 0;
@@ -1122,6 +1129,13 @@ void dual_kernel(DualArgs a)
 //              the solver's H is that of phi / scale, so H_s x = (V^T g V) / scale
 //     K      = N lam N  ->  Kr (slot S_R), Ki (slot S_Z0), original basis
 // The streaming kernel then forms dL/dB = G_c N + 2i B K.
+template <int NT>
+__global__ __launch_bounds__(DT)
+void dual_kernel(DualArgs a)
+{
+    dual_body<NT>(a, blockIdx.x);
+}
+
 struct AdjArgs {
     double* ws;                    // solver workspace, one target_doubles(mp) block per target
     const double* Pf; const double* Qf;   // [b][mp][mp] N = Pf + i Qf, original basis
@@ -1341,6 +1355,82 @@ hipError_t launch_dual(hipStream_t stream, const double* Cr, const double* Ci, l
     if (err != hipSuccess) return err;
     return launch_dual_newton(stream, Cr, Ci, sC_b, ldc, batch, m, omega, sO_b, ldo, Pf, Qf,
                               iters, status, counters, ws, lds_limit);
+}
+
+// ---------------------------------------------------------------- uneven batches
+// One workgroup per target of the plan; each runs the body of its own order / eigensolver
+// variant. The dynamic LDS of a launch is the largest of its targets' needs.
+__global__ __launch_bounds__(ET)
+void eigh_uneven_kernel(const UTarget* __restrict__ tg)
+{
+    const UTarget& t = tg[blockIdx.x];
+    EighArgs a;
+    a.Cr = t.Cr; a.sC_b = 0; a.ldc = t.m; a.m = t.m; a.mp = t.mp; a.ws = t.dual; a.v_in_lds = t.eigh_vlds;
+    if (t.eigh_small) {
+        if (t.eigh_vlds) eigh_body<JB_BATCH, true>(a, 0); else eigh_body<JB_BATCH, false>(a, 0);
+    } else {
+        if (t.eigh_vlds) eigh_body<JB_MAX, true>(a, 0); else eigh_body<JB_MAX, false>(a, 0);
+    }
+}
+
+__global__ __launch_bounds__(DT)
+void dual_uneven_kernel(const UTarget* __restrict__ tg)
+{
+    const UTarget& t = tg[blockIdx.x];
+    DualArgs a;
+    a.Cr = t.Cr; a.Ci = t.Ci; a.sC_b = 0; a.ldc = t.m; a.m = t.m; a.mp = t.mp;
+    a.omega = t.omega; a.sO_b = 0; a.ldo = (int)t.ldo; a.Pf = t.Pf; a.Qf = t.Qf;
+    a.iters = t.iters; a.status = t.status; a.counters = t.counters; a.ws = t.dual; a.v_in_lds = 0;
+    switch (t.mp / 16) {
+        case 1: dual_body<1>(a, 0); break;
+        case 2: dual_body<2>(a, 0); break;
+        case 3: dual_body<3>(a, 0); break;
+        case 4: dual_body<4>(a, 0); break;
+        case 5: dual_body<5>(a, 0); break;
+        case 6: dual_body<6>(a, 0); break;
+        case 7: dual_body<7>(a, 0); break;
+        case 8: dual_body<8>(a, 0); break;
+        default: break;
+    }
+}
+
+// eigensolver variant of order m on a device with lds_limit bytes of LDS per workgroup
+void eigh_variant(int m, size_t lds_limit, int* small, int* vlds, size_t* lds_bytes) {
+    const int np = m / 2;
+    const int blocks = (np * (np - 1) / 2 + JU - 1) / JU;
+    const int rows = (m + JU / np - 1) / (JU / np);
+    *small = (blocks <= JB_BATCH && rows <= JV_SMALL) ? 1 : 0;
+    *vlds = eigh_lds_doubles(m, true) * 8 <= lds_limit ? 1 : 0;
+    *lds_bytes = eigh_lds_doubles(m, *vlds != 0) * 8;
+}
+
+size_t dual_lds_bytes(int m) {
+    switch (pad16(m) / 16) {
+        case 1: return (size_t)Cfg<1>::LDS_END * 8;
+        case 2: return (size_t)Cfg<2>::LDS_END * 8;
+        case 3: return (size_t)Cfg<3>::LDS_END * 8;
+        case 4: return (size_t)Cfg<4>::LDS_END * 8;
+        case 5: return (size_t)Cfg<5>::LDS_END * 8;
+        case 6: return (size_t)Cfg<6>::LDS_END * 8;
+        case 7: return (size_t)Cfg<7>::LDS_END * 8;
+        case 8: return (size_t)Cfg<8>::LDS_END * 8;
+        default: return 0;
+    }
+}
+
+hipError_t launch_dual_uneven(hipStream_t stream, const UTarget* tg, int count, size_t eigh_lds, size_t dual_lds)
+{
+    hipError_t err = hipFuncSetAttribute((const void*)eigh_uneven_kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)eigh_lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(eigh_uneven_kernel, dim3(count), dim3(ET), eigh_lds, stream, tg);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    err = hipFuncSetAttribute((const void*)dual_uneven_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)dual_lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(dual_uneven_kernel, dim3(count), dim3(DT), dual_lds, stream, tg);
+    return hipGetLastError();
 }
 
 }  // namespace ssn

@@ -12,6 +12,7 @@
 // products); all tiles of A^T D are formed and the skew part is taken in the
 // deterministic second stage, which sums the chunk partials in a fixed order.
 #include "common.h"
+#include "uneven.h"
 
 namespace ssn {
 
@@ -46,17 +47,16 @@ __device__ __forceinline__ void tile_of(int t, int nt, int ncr, int& I, int& J, 
 
 // partial layout: [batch][chunks][2][MP][MP]; slot 0 = A^T A + D^T D (upper tiles),
 // slot 1 = A^T D (all tiles).
+// Body of one (row chunk, target) workgroup; `stage` is 2 * GramShape<NT>::STAGE doubles of LDS.
 template <int NT>
-__global__ __launch_bounds__(GRAM_THREADS)
-void gram_partial_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
-                         int n, int m, int chunk_rows, int chunks,
-                         double* __restrict__ partial)
+__device__ __forceinline__ void gram_partial_body(const double* __restrict__ T, long long sT_b, long long sT_r,
+                                                  int n, int m, int chunk_rows, int chunks,
+                                                  double* __restrict__ partial, int chunk, int b,
+                                                  double* __restrict__ stage_lds)
 {
     using S = GramShape<NT>;
     constexpr int MP = S::MP;
-    __shared__ double stage[2][S::STAGE];
 
-    const int chunk = blockIdx.x, b = blockIdx.y;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, q = lane >> 4;
     const int row_begin = chunk * chunk_rows;
@@ -112,7 +112,7 @@ void gram_partial_kernel(const double* __restrict__ T, long long sT_b, long long
     const int nstages = (row_end - row_begin + GRAM_ROWS - 1) / GRAM_ROWS;
     if (nstages > 0) load_stage(row_begin);
     for (int st = 0; st < nstages; ++st) {
-        double* cur = stage[st & 1];
+        double* cur = stage_lds + (st & 1) * S::STAGE;
         store_stage(cur);
         __syncthreads();
         if (st + 1 < nstages) load_stage(row_begin + (st + 1) * GRAM_ROWS);
@@ -152,19 +152,24 @@ void gram_partial_kernel(const double* __restrict__ T, long long sT_b, long long
     }
 }
 
+template <int NT>
+__global__ __launch_bounds__(GRAM_THREADS)
+void gram_partial_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
+                         int n, int m, int chunk_rows, int chunks,
+                         double* __restrict__ partial)
+{
+    __shared__ double stage[2 * GramShape<NT>::STAGE];
+    gram_partial_body<NT>(T, sT_b, sT_r, n, m, chunk_rows, chunks, partial, blockIdx.x, blockIdx.y, stage);
+}
+
 // Deterministic second stage: Cr, Ci [batch][m][m] (leading dimension ldc) from the
 // chunk partials, summed in ascending chunk order. Cr is mirrored from its upper
 // triangle, Ci = S - S^T from the A^T D sums, so symmetry / skewness is exact.
-__global__ void gram_reduce_kernel(const double* __restrict__ partial, int chunks, int m, int mp,
-                                   double* __restrict__ Cr, double* __restrict__ Ci,
-                                   long long sC_b, int ldc)
+__device__ __forceinline__ void gram_reduce_element(const double* __restrict__ p, int chunks, int m, int mp,
+                                                    double* __restrict__ Crb, double* __restrict__ Cib, int ldc, int e)
 {
-    const int b = blockIdx.y;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= m * m) return;
     const int i = e / m, j = e - (e / m) * m;
     const int lo = min(i, j), hi = max(i, j);
-    const double* p = partial + (long long)b * chunks * 2 * mp * mp;
     double cr = 0.0, ad = 0.0, da = 0.0;
     for (int c = 0; c < chunks; ++c) {
         const double* pc = p + (long long)c * 2 * mp * mp;
@@ -173,8 +178,42 @@ __global__ void gram_reduce_kernel(const double* __restrict__ partial, int chunk
         da += pc[mp * mp + hi * mp + lo];
     }
     const double ci_upper = ad - da;   // Ci[lo][hi]
-    Cr[(long long)b * sC_b + (long long)i * ldc + j] = cr;
-    Ci[(long long)b * sC_b + (long long)i * ldc + j] = (i == j) ? 0.0 : (i < j ? ci_upper : -ci_upper);
+    Crb[(long long)i * ldc + j] = cr;
+    Cib[(long long)i * ldc + j] = (i == j) ? 0.0 : (i < j ? ci_upper : -ci_upper);
+}
+
+__global__ void gram_reduce_kernel(const double* __restrict__ partial, int chunks, int m, int mp,
+                                   double* __restrict__ Cr, double* __restrict__ Ci,
+                                   long long sC_b, int ldc)
+{
+    const int b = blockIdx.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * m) return;
+    gram_reduce_element(partial + (long long)b * chunks * 2 * mp * mp, chunks, m, mp,
+                        Cr + (long long)b * sC_b, Ci + (long long)b * sC_b, ldc, e);
+}
+
+// ---------------------------------------------------------------- uneven batches
+// Persistent grid over the (target, chunk) items of one order of a plan (largest first).
+template <int NT>
+__global__ __launch_bounds__(GRAM_THREADS)
+void gram_uneven_kernel(const UTarget* __restrict__ tg, const UItem* __restrict__ items, int nitems)
+{
+    __shared__ double stage[2 * GramShape<NT>::STAGE];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const UItem w = items[it];
+        const UTarget& t = tg[w.target];
+        gram_partial_body<NT>(t.T, 0, t.sT_r, t.n, t.m, t.chunk_rows, t.chunks, t.partial, w.index, 0, stage);
+        __syncthreads();      // the next item restages the LDS buffers
+    }
+}
+
+// One workgroup per target: Cr, Ci (ld m) from its chunk partials.
+__global__ void gram_reduce_uneven_kernel(const UTarget* __restrict__ tg)
+{
+    const UTarget& t = tg[blockIdx.x];
+    for (int e = threadIdx.x; e < t.m * t.m; e += blockDim.x)
+        gram_reduce_element(t.partial, t.chunks, t.m, t.mp, t.Cr, t.Ci, t.m, e);
 }
 
 // ---------------------------------------------------------------- cross Gram (backward)
@@ -375,6 +414,28 @@ hipError_t launch_gram_reduce(hipStream_t stream, const double* partial, int bat
     dim3 grid((m * m + 255) / 256, batch), block(256);
     hipLaunchKernelGGL(gram_reduce_kernel, grid, block, 0, stream, partial, chunks, m, mp,
                        Cr, Ci, sC_b, ldc);
+    return hipGetLastError();
+}
+
+hipError_t launch_gram_uneven(hipStream_t stream, const UTarget* tg, const UItem* items, const int* begin,
+                              int count)
+{
+    for (int nt = 1; nt <= 8; ++nt) {
+        const int first = begin[nt - 1], nitems = begin[nt] - first;
+        if (nitems <= 0) continue;
+        const int grid = nitems < 512 ? nitems : 512;     // two resident workgroups per CU
+#define SSN_GU_CASE(NTV) \
+        case NTV: hipLaunchKernelGGL(gram_uneven_kernel<NTV>, dim3(grid), dim3(GRAM_THREADS), 0, stream, \
+                                     tg, items + first, nitems); break;
+        switch (nt) {
+            SSN_GU_CASE(1) SSN_GU_CASE(2) SSN_GU_CASE(3) SSN_GU_CASE(4)
+            SSN_GU_CASE(5) SSN_GU_CASE(6) SSN_GU_CASE(7) SSN_GU_CASE(8)
+        }
+#undef SSN_GU_CASE
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(gram_reduce_uneven_kernel, dim3(count), dim3(256), 0, stream, tg);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@
 // `build_recovery_fragments`), so every B-operand load of a wave is one
 // contiguous 512-byte read that all workgroups of the target share through L2.
 #include "common.h"
+#include "uneven.h"
 
 namespace ssn {
 
@@ -19,6 +20,21 @@ constexpr int REC_ROWS = REC_WAVES * 16;    // rows of A (and of D) per workgrou
 
 // frag[b][ks][J][lane] = M[4 ks + (lane >> 4)][16 J + (lane & 15)], ks < 2mp/4, J < 2mp/16.
 // P, Q are [batch][mp][mp] row-major with zero padding.
+// Element e (< 4 mp^2) of one target's fragment-ordered recovery matrix.
+__device__ __forceinline__ void recovery_fragment_element(const double* __restrict__ Pb, const double* __restrict__ Qb,
+                                                          int mp, double* __restrict__ fb, int e)
+{
+    const int m2 = 2 * mp;
+    const int lane = e & 63;
+    const int J = (e >> 6) % (m2 / 16);
+    const int ks = (e >> 6) / (m2 / 16);
+    const int k = 4 * ks + (lane >> 4), j = 16 * J + (lane & 15);
+    double v;
+    if (k < mp) v = (j < mp) ? Pb[j * mp + k] : -Qb[(j - mp) * mp + k];           // P^T | -Q^T
+    else        v = (j < mp) ? Qb[j * mp + (k - mp)] : Pb[(j - mp) * mp + (k - mp)]; // Q^T |  P^T
+    fb[e] = v;
+}
+
 __global__ void build_recovery_fragments(const double* __restrict__ P, const double* __restrict__ Q,
                                          int mp, double* __restrict__ frag)
 {
@@ -26,36 +42,28 @@ __global__ void build_recovery_fragments(const double* __restrict__ P, const dou
     const int m2 = 2 * mp;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m2 * m2) return;
-    const int lane = e & 63;
-    const int J = (e >> 6) % (m2 / 16);
-    const int ks = (e >> 6) / (m2 / 16);
-    const int k = 4 * ks + (lane >> 4), j = 16 * J + (lane & 15);
-    const double* Pb = P + (long long)b * mp * mp;
-    const double* Qb = Q + (long long)b * mp * mp;
-    double v;
-    if (k < mp) v = (j < mp) ? Pb[j * mp + k] : -Qb[(j - mp) * mp + k];           // P^T | -Q^T
-    else        v = (j < mp) ? Qb[j * mp + (k - mp)] : Pb[(j - mp) * mp + (k - mp)]; // Q^T |  P^T
-    frag[(long long)b * m2 * m2 + e] = v;
+    recovery_fragment_element(P + (long long)b * mp * mp, Q + (long long)b * mp * mp, mp,
+                              frag + (long long)b * m2 * m2, e);
 }
 
 // Streaming product of the rows of up to two 2n x m inputs with a fragment-ordered matrix:
 //   [out_top | out_bot](r) = [a_r | d_r (| gt_r | gb_r)] * M,   M of size (2 NIN mp) x 2mp,
 // where a/d (gt/gb) are the top/bottom halves of T (G). NIN = 1 is the recovery, NIN = 2 the
 // backward product dL/dT = [Re; Im](G_c N + 2i B K).
+// Body of one workgroup of REC_THREADS threads on rows [row0, row0 + REC_ROWS) of target b.
 template <int NT, int NIN>
-__global__ __launch_bounds__(REC_THREADS)
-void stream_gemm_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
-                        const double* __restrict__ G, long long sG_b, long long sG_r,
-                        const double* __restrict__ frag, int n, int m,
-                        double* __restrict__ X, long long sX_b, long long sX_r)
+__device__ __forceinline__ void stream_gemm_body(const double* __restrict__ T, long long sT_b, long long sT_r,
+                                                 const double* __restrict__ G, long long sG_b, long long sG_r,
+                                                 const double* __restrict__ frag, int n, int m,
+                                                 double* __restrict__ X, long long sX_b, long long sX_r,
+                                                 int row0, int b)
 {
     constexpr int MP = NT * 16;
     constexpr int NJ = 2 * NT;            // output tiles per row block (top + bottom)
     constexpr int KS = 2 * NIN * MP / 4;  // k-steps
-    const int b = blockIdx.y;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = lane & 15, q = lane >> 4;
-    const int row = blockIdx.x * REC_ROWS + wave * 16 + r;      // A-operand row of this lane
+    const int row = row0 + wave * 16 + r;      // A-operand row of this lane
     const bool row_ok = row < n;
     const double* Tb = T + (long long)b * sT_b;
     const double* Arow = Tb + (long long)row * sT_r;
@@ -96,11 +104,22 @@ void stream_gemm_kernel(const double* __restrict__ T, long long sT_b, long long 
         if (col >= m) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int orow = blockIdx.x * REC_ROWS + wave * 16 + q + 4 * i;
+            const int orow = row0 + wave * 16 + q + 4 * i;
             if (orow < n)
                 Xb[(long long)(top ? orow : orow + n) * sX_r + col] = acc[j][i];
         }
     }
+}
+
+template <int NT, int NIN>
+__global__ __launch_bounds__(REC_THREADS)
+void stream_gemm_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
+                        const double* __restrict__ G, long long sG_b, long long sG_r,
+                        const double* __restrict__ frag, int n, int m,
+                        double* __restrict__ X, long long sX_b, long long sX_r)
+{
+    stream_gemm_body<NT, NIN>(T, sT_b, sT_r, G, sG_b, sG_r, frag, n, m, X, sX_b, sX_r,
+                              blockIdx.x * REC_ROWS, blockIdx.y);
 }
 
 // Backward fragments: M_bwd (4mp x 2mp) with row blocks
@@ -186,6 +205,48 @@ hipError_t launch_backward_stream(hipStream_t stream, const double* T, long long
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_stream<2>(stream, T, sT_b, sT_r, G, sG_b, sG_r, frag, batch, n, m, dT, sD_b, sD_r);
+}
+
+// ---------------------------------------------------------------- uneven batches
+__global__ void recovery_fragments_uneven_kernel(const UTarget* __restrict__ tg)
+{
+    const UTarget& t = tg[blockIdx.y];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < 4 * t.mp * t.mp) recovery_fragment_element(t.Pf, t.Qf, t.mp, t.frag, e);
+}
+
+// Persistent grid over the (target, first row) items of a plan: each item is rec_rows rows
+// (a multiple of REC_ROWS) of one target, streamed REC_ROWS at a time.
+__global__ __launch_bounds__(REC_THREADS)
+void recover_uneven_kernel(const UTarget* __restrict__ tg, const UItem* __restrict__ items, int nitems)
+{
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const UItem w = items[it];
+        const UTarget& t = tg[w.target];
+        for (int row0 = w.index; row0 < w.index + t.rec_rows && row0 < t.n; row0 += REC_ROWS) {
+            switch (t.mp / 16) {
+#define SSN_RU_CASE(NTV) \
+                case NTV: stream_gemm_body<NTV, 1>(t.T, 0, t.sT_r, nullptr, 0, 0, t.frag, t.n, t.m, t.X, 0, t.sX_r, \
+                                                   row0, 0); break;
+                SSN_RU_CASE(1) SSN_RU_CASE(2) SSN_RU_CASE(3) SSN_RU_CASE(4)
+                SSN_RU_CASE(5) SSN_RU_CASE(6) SSN_RU_CASE(7) SSN_RU_CASE(8)
+#undef SSN_RU_CASE
+                default: break;
+            }
+        }
+    }
+}
+
+hipError_t launch_recover_uneven(hipStream_t stream, const UTarget* tg, const UItem* items, int nitems,
+                                 int count, int max_mp)
+{
+    hipLaunchKernelGGL(recovery_fragments_uneven_kernel, dim3((4 * max_mp * max_mp + 255) / 256, count), dim3(256),
+                       0, stream, tg);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nitems == 0) return e;
+    const int grid = nitems < 1024 ? nitems : 1024;
+    hipLaunchKernelGGL(recover_uneven_kernel, dim3(grid), dim3(REC_THREADS), 0, stream, tg, items, nitems);
+    return hipGetLastError();
 }
 
 }  // namespace ssn

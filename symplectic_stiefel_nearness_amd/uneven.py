@@ -1,0 +1,131 @@
+"""Uneven batches: targets of mixed (n, k) solved by one launch per stage, optionally captured
+in a HIP graph and replayed.
+
+The C ABI plans the batch (workspace regions and work lists, largest work first), the plan
+is copied to the start of the device workspace, and ssn_solve_uneven launches the Gram
+(one persistent launch per order), the eigensolver, the Newton iteration (one workgroup per
+target) and the recovery (one persistent launch). Tile shapes per target (Gram rows per work
+item, recovery rows per work item) come from `tuning`, e.g. the autotuner's table.
+No reference counterpart: the reference solves one (n, k) per call
+(dual_recommended_hvp.py:231-242); each target's result is the one `solve` gives for it.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .dual import _raise_for_status, _require_ready, _stream, _workspace
+
+DEFAULT_TILE = (512, 64)     # (Gram rows per work item, recovery rows per work item)
+
+
+def _tile(tuning, n, m):
+    if not tuning:
+        return DEFAULT_TILE
+    return tuning.get((n, m), DEFAULT_TILE)
+
+
+class UnevenSolver:
+    """Static buffers, plan and (optionally) a captured graph for one list of target shapes.
+
+    shapes: list of (n, k); targets are [2n, 2k] float64. run(targets) copies the targets into
+    the solver's input buffers and returns [(X, omega, iterations)], views of the solver's
+    output buffers (overwritten by the next run).
+    """
+
+    def __init__(self, shapes, device=None, tuning=None, capture=False, inputs=None):
+        self.lib = _require_ready()
+        dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
+        self.shapes = [(int(n), int(k)) for n, k in shapes]
+        count = len(self.shapes)
+        assert count > 0
+        f64 = dict(dtype=torch.float64, device=dev)
+        if inputs is None:
+            self.T = [torch.zeros(2 * n, 2 * k, **f64) for n, k in self.shapes]
+        else:
+            self.T = list(inputs)
+        self.X = [torch.empty(2 * n, 2 * k, **f64) for n, k in self.shapes]
+        self.omega = [torch.empty(2 * k, 2 * k, **f64) for n, k in self.shapes]
+        self.iters = torch.zeros(count, dtype=torch.int64, device=dev)
+        self.status = torch.zeros(count, dtype=torch.int32, device=dev)
+        self.counters = torch.zeros((count, 12), dtype=torch.int32, device=dev)
+        arr = (_lib.SsnTarget * count)()
+        for i, (n, k) in enumerate(self.shapes):
+            t = self.T[i]
+            assert t.shape == (2 * n, 2 * k) and t.dtype == torch.float64 and t.device == dev
+            assert t.stride(1) == 1, "targets must have unit column stride"
+            cr, rr = _tile(tuning, n, 2 * k)
+            arr[i] = _lib.SsnTarget(t.data_ptr(), t.stride(0), self.X[i].data_ptr(), self.X[i].stride(0),
+                                    self.omega[i].data_ptr(), self.omega[i].stride(0),
+                                    self.iters.data_ptr() + 8 * i, self.status.data_ptr() + 4 * i,
+                                    self.counters.data_ptr() + 48 * i, n, 2 * k, cr, rr)
+        self.targets = arr
+        nbytes = self.lib.ssn_uneven_workspace_bytes(arr, count)
+        if nbytes == 0:
+            raise ValueError("unsupported target shape in the uneven batch (k must be 1..64)")
+        self.ws = _workspace(nbytes, dev)
+        pbytes = self.lib.ssn_uneven_plan_bytes(arr, count)
+        self.plan = (ctypes.c_uint8 * pbytes)()
+        with torch.cuda.device(dev):
+            rc = self.lib.ssn_uneven_plan(arr, count, self.ws.data_ptr(), self.ws.numel(),
+                                          ctypes.addressof(self.plan), pbytes)
+        _lib.check(rc, "ssn_uneven_plan")
+        host = torch.frombuffer(bytearray(bytes(self.plan)), dtype=torch.uint8)
+        self.ws[:pbytes].copy_(host)
+        self.graph = None
+        if capture:
+            self._capture()
+
+    def _launch(self):
+        rc = self.lib.ssn_solve_uneven(_stream(), ctypes.addressof(self.plan), self.ws.data_ptr(),
+                                       self.ws.numel())
+        _lib.check(rc, "ssn_solve_uneven")
+
+    def _capture(self):
+        with torch.cuda.device(self.device):
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                self._launch()                      # warm-up outside the graph
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self._launch()
+            torch.cuda.synchronize()
+
+    def run(self, targets=None, check=True):
+        if targets is not None:
+            assert len(targets) == len(self.T)
+            for dst, src in zip(self.T, targets):
+                dst.copy_(src)
+        with torch.cuda.device(self.device):
+            if self.graph is not None:
+                self.graph.replay()
+            else:
+                self._launch()
+        if check:
+            _raise_for_status(self.status)
+        return [(self.X[i], self.omega[i], self.iters[i:i + 1]) for i in range(len(self.T))]
+
+
+@torch.no_grad()
+def solve_uneven(targets, tuning=None, return_stats=False):
+    """Solve a list of float64 targets [2n_i, 2k_i] (device tensors) of mixed shapes.
+
+    Returns [(X_i, omega_i, iterations_i)]; with return_stats also dict(status, counters).
+    Raises as `solve` does (RuntimeError for a line-search exhaustion or the Newton cap).
+    """
+    targets = [t if t.stride(-1) == 1 else t.contiguous() for t in targets]
+    assert targets and all(t.is_cuda and t.dtype == torch.float64 and t.ndim == 2 for t in targets)
+    shapes = [(t.shape[0] // 2, t.shape[1] // 2) for t in targets]
+    solver = UnevenSolver(shapes, device=targets[0].device, tuning=tuning, inputs=targets)
+    out = solver.run(check=False)
+    _raise_for_status(solver.status)
+    out = [(X.clone(), om.clone(), it.clone()) for X, om, it in out]
+    if return_stats:
+        return out, dict(status=solver.status.clone(), counters=solver.counters.clone())
+    return out
