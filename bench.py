@@ -68,6 +68,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-reps", type=int, default=1)
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with one 256 MiB device-to-device copy (scale for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the solve shards by "
@@ -196,6 +199,32 @@ def main():
     log(f"fwd+bwd: fp64 {fb[False]:.3f} ms, split forward {fb[True]:.3f} ms (backward alone {bwd_ms:.3f} ms, "
         f"adjoint CG {adj_it:.1f} it)")
 
+    # uneven batch: 128 seeded targets of mixed (n, k), eager launches and graph replays
+    uneven = None
+    if not args.no_uneven:
+        Tu = [t.to(dev) for t in inputs.make_uneven(128)]
+        solver = ssn.UnevenSolver([(t.shape[0] // 2, t.shape[1] // 2) for t in Tu], inputs=Tu)
+        solver.run(check=False)
+        torch.cuda.synchronize()
+        t5 = time.perf_counter()
+        for _ in range(3):
+            solver.run(check=False)
+        torch.cuda.synchronize()
+        eager_ms = (time.perf_counter() - t5) * 1e3 / 3
+        solver_g = ssn.UnevenSolver([(t.shape[0] // 2, t.shape[1] // 2) for t in Tu], inputs=Tu, capture=True)
+        solver_g.run(check=False)
+        torch.cuda.synchronize()
+        t6 = time.perf_counter()
+        for _ in range(3):
+            solver_g.run(check=False)
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - t6) * 1e3 / 3
+        ok = bool((solver_g.status == 0).all().item())
+        uneven = {"targets": 128, "eager_ms": eager_ms, "graph_ms": graph_ms,
+                  "solves_per_s": 128 / (graph_ms * 1e-3), "all_status_ok": ok}
+        log(f"uneven 128: eager {eager_ms:.2f} ms, graph {graph_ms:.2f} ms")
+        del Tu, solver, solver_g
+
     # reference protocol boundary: CPU tensor in -> CPU tensors out (PCIe included)
     ssn.solve(T_cpu)
     torch.cuda.synchronize()
@@ -263,6 +292,7 @@ def main():
         "fwd_bwd_solves_per_s": BATCH / (fb_ms * 1e-3),
         "bwd_ms": bwd_ms,
         "adjoint": {"cg_mean": adj_it, "all_converged": adj_ok},
+        "uneven_128": uneven,
         "stage_ms": stages,
         "dual_phase_ms": eigh_ms + newton_ms,
         "forward_bytes_per_step": bytes_fwd,
@@ -281,6 +311,13 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+    if args.calibrate_copy:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 268435456 bytes read, 268435456 bytes written")
 
 
 if __name__ == "__main__":
