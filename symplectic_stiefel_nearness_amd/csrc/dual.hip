@@ -176,7 +176,7 @@ __device__ __forceinline__ void for_tiles(const Ctx& c, const WT& w, const Acc<N
 //   ga(t, i, k) -> op(A_t)[i][k],  gb(t, k, j) -> op(B_t)[k][j]  (global memory).
 // TA / TB say which index is contiguous in memory, for coalesced staging.
 template <int NT, bool TA, bool TB, class GA, class GB>
-__device__ void gemm_acc(const Ctx& c, double* lds, int nterms, GA ga, GB gb, Acc<NT>& acc, const WT& w) {
+__device__ __forceinline__ void gemm_acc(const Ctx& c, double* lds, int nterms, GA ga, GB gb, Acc<NT>& acc, const WT& w) {
     using C = Cfg<NT>;
     constexpr int MP = C::MP, KP = C::KP, PER = C::PER;
     constexpr int PPT = MP / KP;            // panels per term
@@ -257,7 +257,7 @@ __device__ void gemm_acc(const Ctx& c, double* lds, int nterms, GA ga, GB gb, Ac
 
 // C = op(A) op(B) with global row-major MP x MP operands; epilogue f(row, col, value).
 template <int NT, bool TA, bool TB, class F>
-__device__ void gemm(const Ctx& c, double* lds, const double* A, const double* B, F f) {
+__device__ __forceinline__ void gemm(const Ctx& c, double* lds, const double* A, const double* B, F f) {
     constexpr int MP = Cfg<NT>::MP;
     const WT w = wave_tiles<NT>(c.wave);
     Acc<NT> acc; acc.zero();
@@ -270,7 +270,7 @@ __device__ void gemm(const Ctx& c, double* lds, const double* A, const double* B
 
 // C = A1 B1 + s A2 B2 (non-transposed operands), epilogue f.
 template <int NT, class F>
-__device__ void gemm2(const Ctx& c, double* lds, const double* A1, const double* B1,
+__device__ __forceinline__ void gemm2(const Ctx& c, double* lds, const double* A1, const double* B1,
                       const double* A2, const double* B2, double s, F f) {
     constexpr int MP = Cfg<NT>::MP;
     const WT w = wave_tiles<NT>(c.wave);
@@ -284,7 +284,7 @@ __device__ void gemm2(const Ctx& c, double* lds, const double* A1, const double*
 
 // C = A1 B1^T + s A2 B2^T, epilogue f.
 template <int NT, class F>
-__device__ void gemm2_nt(const Ctx& c, double* lds, const double* A1, const double* B1,
+__device__ __forceinline__ void gemm2_nt(const Ctx& c, double* lds, const double* A1, const double* B1,
                          const double* A2, const double* B2, double s, F f) {
     constexpr int MP = Cfg<NT>::MP;
     const WT w = wave_tiles<NT>(c.wave);
@@ -865,6 +865,265 @@ void eigh_kernel(EighArgs a)
     eigh_body<JB, V_IN_LDS>(a, blockIdx.x);
 }
 
+// ---------------------------------------------------------------- chained-HVP PCG
+// The preconditioned CG of the Newton step (dual_recommended_hvp.py:141-162, with the HVP of
+// :74-76 = hvp_fused.cu:1-148) with the whole iteration held on chip, for even NT <= 6:
+//   * the search direction p (skew) lives in LDS as a full MP x LS matrix;
+//   * the HVP H(p) = Z - Z^T, Z = P (p Sr) - Q (p Si), runs as chained MFMA products: a wave
+//     owns column block J of one or both terms; its first product U[:, J] = p S_t[:, J] ends
+//     in the accumulator layout (lane r + 16 q holds U[16 I + q + 4 e][16 J + r]), which is
+//     exactly the B-operand layout of k-block 4 I + e of the second product, so
+//     Z[:, J] = M_t U[:, J] (M_0 = P, M_1 = -Q) takes U from registers. The items are
+//     (column block, term) pairs, three per SIMD at NT = 6;
+//   * Z goes to LDS (over p, after every wave has finished reading p) for the transpose;
+//   * all CG iterates are exactly skew, so each thread owns strictly upper elements: the
+//     direction in registers, the residual and the solution in LDS arrays indexed by the
+//     element number; the inner products are twice the upper sums.
+// P is exactly symmetric (mirrored inverse product), so its A fragment P[16 I + r][4 kb + q]
+// is read as P[4 kb + q][16 I + r], a coalesced row segment; Q is read as stored.
+template <int NT>
+struct ChainCfg {
+    static constexpr int MP = 16 * NT;
+    static constexpr int LS = MP + 1;               // LDS row stride (odd: conflict-free columns)
+    static constexpr int UN = MP * (MP - 1) / 2;    // strictly upper elements
+    static constexpr int UPT = (UN + DT - 1) / DT;  // owned per thread
+    static constexpr int OFF = 48;                  // after the reduction scratch
+    static constexpr int OFF_R = OFF + MP * LS;     // residual, then solution, UN doubles each
+    static constexpr int OFF_X = OFF_R + UN;
+    static constexpr bool OK = (NT == 2 || NT == 4 || NT == 6);
+};
+static_assert(ChainCfg<6>::OFF_X + ChainCfg<6>::UN <= Cfg<6>::OFF_LAM, "chained CG LDS overlaps lambda");
+static_assert(ChainCfg<4>::OFF_X + ChainCfg<4>::UN <= Cfg<4>::OFF_LAM, "chained CG LDS overlaps lambda");
+static_assert(ChainCfg<2>::OFF_X + ChainCfg<2>::UN <= Cfg<2>::OFF_LAM, "chained CG LDS overlaps lambda");
+
+// Column block J and term mask (bit t: term t) of a wave's chained-HVP items.
+template <int NT>
+__device__ __forceinline__ void chain_items(int wave, int& J, int& tmask) {
+    if (NT == 6) {
+        if (wave < 4) { J = wave; tmask = 3; }
+        else { J = 4 + (wave & 1); tmask = 1 << ((wave >> 1) & 1); }
+    } else if (NT == 4) {
+        J = wave & 3; tmask = 1 << (wave >> 2);
+    } else {   // NT == 2
+        J = wave & 1; tmask = wave < 4 ? 1 << (wave >> 1) : 0;
+    }
+}
+
+// Strictly upper element u in [0, MP (MP - 1) / 2): rows a and MP - 1 - a fold into one
+// row of MP - 1 elements.
+template <int MP>
+__device__ __forceinline__ void upper_ij(int u, int& i, int& j) {
+    const int a = u / (MP - 1), b = u - a * (MP - 1);
+    if (b < MP - 1 - a) { i = a; j = a + 1 + b; } else { i = MP - 1 - a; j = b + 1; }
+}
+
+// Raw buffer resource over n doubles at p (loads past the end return 0).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const double* p, int n) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, n * 8, 0x00020000);
+}
+// One double at byte offset vo (per lane) + so (uniform).
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t rs, int vo, int so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, 0));
+}
+
+// Z[:, J] of one term (t = 0: P (p Sr); t = 1: -Q (p Si)) accumulated into accZ. The
+// operands in global memory are read through buffer resources with one per-lane offset and
+// uniform k / row-block offsets, so no per-load address registers are kept live.
+template <int NT>
+__device__ __forceinline__ void chain_term(int t, int J, int r, int q, const double* M,
+                                           const double* Sr, const double* Si, const double* P, const double* Q,
+                                           d4 (&accZ)[NT]) {
+    constexpr int MP = 16 * NT, LS = MP + 1, K4 = 4 * NT;
+    const __amdgpu_buffer_rsrc_t rS = buf_rsrc(t == 0 ? Sr : Si, MP * MP);
+    const int vS = opaque((q * MP + 16 * J + r) * 8);
+    const double* Mp = M + opaque(r * LS + q);
+    d4 accU[NT];
+#pragma unroll
+    for (int I = 0; I < NT; ++I) accU[I] = zero4();
+    // U[:, J] = p S_t[:, J]: A = p from LDS, B = rows of S_t (coalesced)
+    double bq[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) bq[u] = buf_ld(rS, vS, 4 * u * MP * 8);
+#pragma unroll 1
+    for (int kb0 = 0; kb0 < K4; kb0 += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kb = kb0 + u;
+            const double bcur = bq[u];
+            bq[u] = buf_ld(rS, vS, 4 * (kb + 4) * MP * 8);   // past the end: 0, unused
+            double a[NT];
+#pragma unroll
+            for (int I = 0; I < NT; ++I) a[I] = Mp[16 * I * LS + 4 * kb];
+#pragma unroll
+            for (int I = 0; I < NT; ++I) accU[I] = mfma(a[I], bcur, accU[I]);
+        }
+    }
+    // Z[:, J] += M_t U[:, J]; U[:, J] in registers is the B operand
+    const __amdgpu_buffer_rsrc_t rA = buf_rsrc(t == 0 ? P : Q, MP * MP);
+    const int vA = opaque(t == 0 ? (q * MP + r) * 8 : (r * MP + q) * 8);
+    const int ks = t == 0 ? 4 * MP * 8 : 4 * 8;          // A-fragment step per k-block (bytes)
+    const int is = t == 0 ? 16 * 8 : 16 * MP * 8;        // A-fragment step per row block
+    const double sg = t == 0 ? 1.0 : -1.0;
+    double an[2][NT];
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+#pragma unroll
+        for (int I = 0; I < NT; ++I) an[d][I] = sg * buf_ld(rA, vA, d * ks + I * is);
+#pragma unroll
+    for (int kb = 0; kb < K4; ++kb) {
+        const double bb = accU[kb >> 2][kb & 3];
+#pragma unroll
+        for (int I = 0; I < NT; ++I) accZ[I] = mfma(an[kb & 1][I], bb, accZ[I]);
+        if (kb + 2 < K4) {
+#pragma unroll
+            for (int I = 0; I < NT; ++I) an[kb & 1][I] = sg * buf_ld(rA, vA, (kb + 2) * ks + I * is);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// PCG on the scaled dual Hessian from the initial residual init(i, j) (i < j). Returns the
+// number of Hessian-vector products; `how` = 0 (curvature not positive), 1 (tolerance met),
+// 2 (iteration cap). ADJ selects the adjoint's rule (stop when |r| <= tol |r0|, none if
+// r0 = 0) instead of the forward's (stop unless |r| > tol). Writes the solution (full skew)
+// to X and, if Z0 is given, the preconditioned initial residual (full skew) to Z0.
+template <int NT, bool ADJ, class INIT>
+__device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
+                                          const double* Sr, const double* Si, const double* P, const double* Q,
+                                          const double* lam_s, double tol, double* Xg, double* Z0g, int& how)
+{
+    using K = ChainCfg<NT>;
+    constexpr int MP = K::MP, LS = K::LS, UPT = K::UPT, UN = K::UN;
+    double* M = lds + K::OFF;
+    double* Rl = lds + K::OFF_R;
+    double* Xl = lds + K::OFF_X;
+    const int tid = c.tid, r = c.r, q = c.q;
+    int J, tmask;
+    chain_items<NT>(c.wave, J, tmask);
+    J = __builtin_amdgcn_readfirstlane(J);
+    tmask = __builtin_amdgcn_readfirstlane(tmask);
+
+    double Pd[UPT];
+    double rz = 0.0, rr0 = 0.0;
+    sync();    // earlier users of this LDS area are done
+#pragma unroll
+    for (int s = 0; s < UPT; ++s) {
+        const int u = tid + s * DT;
+        Pd[s] = 0.0;
+        if (u < UN) {
+            int i, j;
+            upper_ij<MP>(u, i, j);
+            const double rv = init(i, j);
+            const double z = rv / (lam_s[i] + lam_s[j]);
+            Rl[u] = rv; Xl[u] = 0.0; Pd[s] = z;
+            rz = fma(rv, z, rz);
+            rr0 = fma(rv, rv, rr0);
+            M[i * LS + j] = z; M[j * LS + i] = -z;
+            if (Z0g) { Z0g[i * MP + j] = z; Z0g[j * MP + i] = -z; }
+        }
+    }
+    for (int t = tid; t < MP; t += DT) {
+        M[t * LS + t] = 0.0;
+        if (Z0g) Z0g[t * MP + t] = 0.0;
+    }
+    wg_sum2(c, rz, rr0);
+    rz *= 2.0; rr0 *= 2.0;
+    const double thr = ADJ ? tol * sqrt(rr0) : tol;
+    int nh = 0;
+    how = 2;
+    if (ADJ && !(sqrt(rr0) > 0.0)) how = 1;
+#pragma unroll 1
+    for (int it = 0; it < MAX_CG && !(ADJ && how == 1); ++it) {
+        ++nh;
+        d4 accZ[NT];
+#pragma unroll
+        for (int I = 0; I < NT; ++I) accZ[I] = zero4();
+        if (tmask & 1) chain_term<NT>(0, J, r, q, M, Sr, Si, P, Q, accZ);
+        if (tmask & 2) chain_term<NT>(1, J, r, q, M, Sr, Si, P, Q, accZ);
+        sync();    // every wave has finished reading p
+        if (tmask == 3 || tmask == 1) {
+#pragma unroll
+            for (int I = 0; I < NT; ++I)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) M[(16 * I + q + 4 * e) * LS + 16 * J + r] = accZ[I][e];
+        }
+        sync();
+        if (tmask == 2) {
+#pragma unroll
+            for (int I = 0; I < NT; ++I)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) M[(16 * I + q + 4 * e) * LS + 16 * J + r] += accZ[I][e];
+        }
+        sync();
+        // H = Z - Z^T on the owned elements; curvature <p, Hp>
+        double H[UPT];
+        double curv = 0.0;
+#pragma unroll
+        for (int s = 0; s < UPT; ++s) {
+            const int u = tid + s * DT;
+            H[s] = 0.0;
+            if (u < UN) {
+                int i, j;
+                upper_ij<MP>(u, i, j);
+                const double h = M[i * LS + j] - M[j * LS + i];
+                H[s] = h;
+                curv = fma(Pd[s], h, curv);
+            }
+        }
+        const double curvature = 2.0 * wg_sum(c, curv);
+        if (!(curvature > 0.0)) { how = 0; break; }
+        const double alpha = rz / curvature;
+        double rr = 0.0, rzn = 0.0;
+#pragma unroll
+        for (int s = 0; s < UPT; ++s) {
+            const int u = tid + s * DT;
+            if (u < UN) {
+                int i, j;
+                upper_ij<MP>(u, i, j);
+                Xl[u] = Xl[u] + alpha * Pd[s];
+                const double rv = Rl[u] - alpha * H[s];
+                Rl[u] = rv;
+                const double z = rv / (lam_s[i] + lam_s[j]);
+                H[s] = z;                       // H now holds the preconditioned residual
+                rr = fma(rv, rv, rr);
+                rzn = fma(rv, z, rzn);
+            }
+        }
+        wg_sum2(c, rr, rzn);    // also: every thread has read Z
+        rr *= 2.0; rzn *= 2.0;
+        if (ADJ ? (sqrt(rr) <= thr) : !(sqrt(rr) > thr)) { how = 1; break; }
+        const double beta = rzn / rz;
+#pragma unroll
+        for (int s = 0; s < UPT; ++s) {
+            const int u = tid + s * DT;
+            if (u < UN) {
+                int i, j;
+                upper_ij<MP>(u, i, j);
+                const double pv = H[s] + beta * Pd[s];
+                Pd[s] = pv;
+                M[i * LS + j] = pv; M[j * LS + i] = -pv;
+            }
+        }
+        for (int t = tid; t < MP; t += DT) M[t * LS + t] = 0.0;
+        rz = rzn;
+        sync();
+    }
+    sync();
+#pragma unroll
+    for (int s = 0; s < UPT; ++s) {
+        const int u = tid + s * DT;
+        if (u < UN) {
+            int i, j;
+            upper_ij<MP>(u, i, j);
+            const double x = Xl[u];
+            Xg[i * MP + j] = x; Xg[j * MP + i] = -x;
+        }
+    }
+    for (int t = tid; t < MP; t += DT) Xg[t * MP + t] = 0.0;
+    sync();
+    return nh;
+}
+
 // ---------------------------------------------------------------- the engine
 // Body of target b's workgroup (DT threads, dynamic LDS Cfg<NT>::LDS_END doubles).
 template <int NT>
@@ -984,6 +1243,14 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
                 Dd[e] = z; Z0[e] = z;
             });
             sync();
+        } else if constexpr (ChainCfg<NT>::OK) {
+            const double tol = fmin(fmax(sqrt(residual), 1e-8), 0.1) * gnorm;
+            const unsigned long long t0 = wall_clock64();
+            int how;
+            const int nh = cg_chained<NT, false>(c, lds, [&](int i, int j) { return -G[i * MP + j]; },
+                                                 Sr, Si, P, Q, lam_s, tol, Dd, Z0, how);
+            n_hvp += nh; n_cg += nh;
+            t_cg += wall_clock64() - t0;
         } else {
             double rz = 0.0, dummy = 0.0;
             each<NT>(c, [&](int i, int j, int e) {
@@ -1196,7 +1463,7 @@ void adjoint_kernel(AdjArgs a)
     int it = 0, status = 1;
     if (!(sqrt(rr0) > 0.0)) status = 0;
 #pragma unroll 1
-    for (; it < MAX_CG && status; ++it) {
+    for (; !ChainCfg<NT>::OK && it < MAX_CG && status; ++it) {
         gemm<NT, false, false>(c, lds, Pp, Sr, [&](int i, int j, double v) { U[i * MP + j] = v; });
         gemm<NT, false, false>(c, lds, Pp, Si, [&](int i, int j, double v) { Vv[i * MP + j] = v; });
         gemm2<NT>(c, lds, Pe, U, Qe, Vv, -1.0, [&](int i, int j, double v) { T2[i * MP + j] = v; });
@@ -1225,6 +1492,13 @@ void adjoint_kernel(AdjArgs a)
         each<NT>(c, [&](int, int, int e) { Pp[e] = T3[e] + beta * Pp[e]; });
         rz = rzn;
         sync();
+    }
+    if constexpr (ChainCfg<NT>::OK) {
+        int how;
+        const int nh = cg_chained<NT, true>(c, lds, [&](int i, int j) { return ((T1[i * MP + j] - T1[j * MP + i]) / 2) / scale; },
+                                            Sr, Si, Pe, Qe, lam_s, a.tol, X, nullptr, how);
+        it = how == 0 ? nh - 1 : nh;
+        status = how == 1 ? 0 : 1;
     }
     // lam = skew(V X V^T) (T3), K = N lam N: Zr = lam Pf (T4), Zi = lam Qf (T5),
     // Kr = Pf Zr - Qf Zi, Ki = Pf Zi + Qf Zr
