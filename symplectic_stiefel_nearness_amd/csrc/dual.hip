@@ -42,7 +42,7 @@ constexpr int NCOUNT = 12;            // counters per target
 // workspace slots, each mp*mp doubles
 enum Slot {
     S_V = 0, S_CIE, S_JSE, S_W, S_P, S_Q, S_SR, S_SI, S_G, S_D, S_R, S_Z0, S_PP, S_HP,
-    S_U, S_VV, S_T1, S_T2, S_T3, S_T4, S_T5, S_COUNT
+    S_U, S_VV, S_T1, S_T2, S_T3, S_T4, S_T5, S_QT, S_COUNT
 };
 static_assert(S_G == 8 && S_D == 9 && S_R == 10 && S_Z0 == 11, "slot numbers used by capi.hip");
 
@@ -180,6 +180,9 @@ __device__ __forceinline__ void gemm_acc(const Ctx& c, double* lds, int nterms, 
     using C = Cfg<NT>;
     constexpr int MP = C::MP, KP = C::KP, PER = C::PER;
     constexpr int PPT = MP / KP;            // panels per term
+    // when the A part of a panel is a whole number of thread rows, element s of every
+    // thread is an A or a B element independently of the thread
+    constexpr bool SPLIT_STATIC = (KP * MP) % DT == 0;
     double* buf0 = lds + C::OFF_PANEL;
     constexpr int DEPTH = C::DEPTH;
     double pre[DEPTH][PER];                 // panels in flight in registers
@@ -194,7 +197,7 @@ __device__ __forceinline__ void gemm_acc(const Ctx& c, double* lds, int nterms, 
         for (int s = 0; s < PER; ++s) {
             const int e = tid + s * DT;
             double v;
-            if (e < KP * MP) {               // A panel element
+            if (SPLIT_STATIC ? (s * DT < KP * MP) : (e < KP * MP)) {   // A panel element
                 int i, kk;
                 if (TA) { kk = e / MP; i = e - kk * MP; } else { i = e / KP; kk = e - i * KP; }
                 v = ga(t, i, k0 + kk);
@@ -211,7 +214,7 @@ __device__ __forceinline__ void gemm_acc(const Ctx& c, double* lds, int nterms, 
 #pragma unroll
         for (int s = 0; s < PER; ++s) {
             const int e = tid + s * DT;
-            if (e < KP * MP) {
+            if (SPLIT_STATIC ? (s * DT < KP * MP) : (e < KP * MP)) {
                 int i, kk;
                 if (TA) { kk = e / MP; i = e - kk * MP; } else { i = e / KP; kk = e - i * KP; }
                 dst[i * C::AS + kk] = src[s];
@@ -880,7 +883,7 @@ void eigh_kernel(EighArgs a)
 //     direction in registers, the residual and the solution in LDS arrays indexed by the
 //     element number; the inner products are twice the upper sums.
 // P is exactly symmetric (mirrored inverse product), so its A fragment P[16 I + r][4 kb + q]
-// is read as P[4 kb + q][16 I + r], a coalesced row segment; Q is read as stored.
+// is read as P[4 kb + q][16 I + r], a coalesced row segment; Q through its transposed copy.
 template <int NT>
 struct ChainCfg {
     static constexpr int MP = 16 * NT;
@@ -926,12 +929,12 @@ __device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t rs, int vo, int 
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, 0));
 }
 
-// Z[:, J] of one term (t = 0: P (p Sr); t = 1: -Q (p Si)) accumulated into accZ. The
+// Z[:, J] of one term (t = 0: P (p Sr); t = 1: -Q (p Si)) accumulated into accZ (QT = Q^T). The
 // operands in global memory are read through buffer resources with one per-lane offset and
 // uniform k / row-block offsets, so no per-load address registers are kept live.
 template <int NT>
 __device__ __forceinline__ void chain_term(int t, int J, int r, int q, const double* M,
-                                           const double* Sr, const double* Si, const double* P, const double* Q,
+                                           const double* Sr, const double* Si, const double* P, const double* QT,
                                            d4 (&accZ)[NT]) {
     constexpr int MP = 16 * NT, LS = MP + 1, K4 = 4 * NT;
     const __amdgpu_buffer_rsrc_t rS = buf_rsrc(t == 0 ? Sr : Si, MP * MP);
@@ -958,25 +961,30 @@ __device__ __forceinline__ void chain_term(int t, int J, int r, int q, const dou
             for (int I = 0; I < NT; ++I) accU[I] = mfma(a[I], bcur, accU[I]);
         }
     }
-    // Z[:, J] += M_t U[:, J]; U[:, J] in registers is the B operand
-    const __amdgpu_buffer_rsrc_t rA = buf_rsrc(t == 0 ? P : Q, MP * MP);
-    const int vA = opaque(t == 0 ? (q * MP + r) * 8 : (r * MP + q) * 8);
-    const int ks = t == 0 ? 4 * MP * 8 : 4 * 8;          // A-fragment step per k-block (bytes)
-    const int is = t == 0 ? 16 * 8 : 16 * MP * 8;        // A-fragment step per row block
-    const double sg = t == 0 ? 1.0 : -1.0;
-    double an[2][NT];
+    // Z[:, J] += M_t U[:, J]; U[:, J] in registers is the B operand. The A fragment
+    // M[16 I + r][4 kb + q] is read from P[4 kb + q][16 I + r] (P exactly symmetric) or from
+    // the transposed copy QT of Q: both are coalesced row segments.
+    const __amdgpu_buffer_rsrc_t rA = buf_rsrc(t == 0 ? P : QT, MP * MP);
+    const int vA = opaque((q * MP + r) * 8);
+    constexpr int ks = 4 * MP * 8;                       // A-fragment step per k-block (bytes)
+    constexpr int is = 16 * 8;                           // A-fragment step per row block
+    // the sign of -Q is carried by the B operand ((-Q) V = Q (-V) product by product), so the
+    // prefetched A fragments are used as loaded
+    constexpr int D = 3;
+    double an[D][NT];
 #pragma unroll
-    for (int d = 0; d < 2; ++d)
+    for (int d = 0; d < D; ++d)
 #pragma unroll
-        for (int I = 0; I < NT; ++I) an[d][I] = sg * buf_ld(rA, vA, d * ks + I * is);
+        for (int I = 0; I < NT; ++I) an[d][I] = buf_ld(rA, vA, d * ks + I * is);
 #pragma unroll
     for (int kb = 0; kb < K4; ++kb) {
-        const double bb = accU[kb >> 2][kb & 3];
+        const double bu = accU[kb >> 2][kb & 3];
+        const double bb = t == 0 ? bu : -bu;
 #pragma unroll
-        for (int I = 0; I < NT; ++I) accZ[I] = mfma(an[kb & 1][I], bb, accZ[I]);
-        if (kb + 2 < K4) {
+        for (int I = 0; I < NT; ++I) accZ[I] = mfma(an[kb % D][I], bb, accZ[I]);
+        if (kb + D < K4) {
 #pragma unroll
-            for (int I = 0; I < NT; ++I) an[kb & 1][I] = sg * buf_ld(rA, vA, (kb + 2) * ks + I * is);
+            for (int I = 0; I < NT; ++I) an[kb % D][I] = buf_ld(rA, vA, (kb + D) * ks + I * is);
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -989,7 +997,7 @@ __device__ __forceinline__ void chain_term(int t, int J, int r, int q, const dou
 // to X and, if Z0 is given, the preconditioned initial residual (full skew) to Z0.
 template <int NT, bool ADJ, class INIT>
 __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
-                                          const double* Sr, const double* Si, const double* P, const double* Q,
+                                          const double* Sr, const double* Si, const double* P, const double* QT,
                                           const double* lam_s, double tol, double* Xg, double* Z0g, int& how)
 {
     using K = ChainCfg<NT>;
@@ -1038,8 +1046,8 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
         d4 accZ[NT];
 #pragma unroll
         for (int I = 0; I < NT; ++I) accZ[I] = zero4();
-        if (tmask & 1) chain_term<NT>(0, J, r, q, M, Sr, Si, P, Q, accZ);
-        if (tmask & 2) chain_term<NT>(1, J, r, q, M, Sr, Si, P, Q, accZ);
+        if (tmask & 1) chain_term<NT>(0, J, r, q, M, Sr, Si, P, QT, accZ);
+        if (tmask & 2) chain_term<NT>(1, J, r, q, M, Sr, Si, P, QT, accZ);
         sync();    // every wave has finished reading p
         if (tmask == 3 || tmask == 1) {
 #pragma unroll
@@ -1124,6 +1132,141 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
     return nh;
 }
 
+// ---------------------------------------------------------------- sandwich in quadrants
+// S = N C N^H in the eigenbasis (dual_recommended_hvp.py:120-129, hvp_sandwich_front
+// hvp_fused.cu:39-80 and the m x m recovery), N = P + iQ, C = diag(lambda) + i Ci, as two
+// rounds of quadrant products with the intermediate in LDS. The products are formed
+// transposed so that every global operand is read as coalesced row segments (P exactly
+// symmetric, Ci read as stored, QT = Q^T):
+//   round A:  Yr^T = diag(lambda) P - Ci^T Q^T,   Yi^T = Ci^T P + diag(lambda) Q^T
+//   round B:  T3^T = P Yr^T + Q Yi^T,             T4^T = P Yi^T - Q Yr^T
+// then Sr = sym(T3), Si = skew(T4), G = J - Si. Each transposed product takes the same
+// products in the same order as the untransposed one, so the results are bitwise those of
+// Yr = P diag - Q Ci, Yi = P Ci + Q diag, T3 = Yr P^T + Yi Q^T, T4 = Yi P^T - Yr Q^T.
+// Wave w computes quadrant (w & 1, (w >> 1) & 1) of output w >> 2 (both outputs of a round
+// have the same work, and waves w, w + 4 share a SIMD).
+template <int NT>
+struct QuadCfg {
+    static constexpr int MP = 16 * NT, H = NT / 2, K4 = 4 * NT;
+    static constexpr int LY = MP + 1;              // LDS row stride of the two m x m outputs
+    static constexpr int OFF = 48;
+    static constexpr int PERE = MP * MP / DT;      // elements per thread in the final pass
+};
+static_assert(QuadCfg<6>::OFF + 2 * QuadCfg<6>::MP * QuadCfg<6>::LY <= Cfg<6>::OFF_LAM, "sandwich LDS");
+static_assert(QuadCfg<4>::OFF + 2 * QuadCfg<4>::MP * QuadCfg<4>::LY <= Cfg<4>::OFF_LAM, "sandwich LDS");
+static_assert(QuadCfg<2>::OFF + 2 * QuadCfg<2>::MP * QuadCfg<2>::LY <= Cfg<2>::OFF_LAM, "sandwich LDS");
+
+// acc[i][j] += sum over k of A(16 (I0 + i) + ., k) B(k, 16 (J0 + j) + .), K = MP, with the A
+// fragment read from global at a[(4 kb + q) MP + 16 (I0 + i) + r] (vA holds the lane and I0
+// part) and the B fragment given by bfrag(kb, j), scaled by bsign.
+template <int NT, class BF>
+__device__ __forceinline__ void quad_mma(__amdgpu_buffer_rsrc_t rA, int vA, double bsign, BF bfrag,
+                                         d4 (&acc)[QuadCfg<NT>::H][QuadCfg<NT>::H]) {
+    constexpr int MP = QuadCfg<NT>::MP, H = QuadCfg<NT>::H, K4 = QuadCfg<NT>::K4;
+    constexpr int D = 2;
+    double ar[D][H];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int i = 0; i < H; ++i) ar[d][i] = buf_ld(rA, vA, (4 * d * MP + 16 * i) * 8);
+#pragma unroll
+    for (int kb = 0; kb < K4; ++kb) {
+        double b[H];
+#pragma unroll
+        for (int j = 0; j < H; ++j) b[j] = bsign * bfrag(kb, j);
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+#pragma unroll
+            for (int j = 0; j < H; ++j) acc[i][j] = mfma(ar[kb % D][i], b[j], acc[i][j]);
+        if (kb + D < K4) {
+#pragma unroll
+            for (int i = 0; i < H; ++i) ar[kb % D][i] = buf_ld(rA, vA, (4 * (kb + D) * MP + 16 * i) * 8);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void sandwich_quad(const Ctx& c, double* lds, const double* Cie, const double* P,
+                                              const double* QT, const double* lam_s, int m, const double* Jse,
+                                              double* Sr, double* Si, double* G, double& gg) {
+    using K = QuadCfg<NT>;
+    constexpr int MP = K::MP, H = K::H, LY = K::LY;
+    double* Y = lds + K::OFF;
+    const int w = c.wave, r = opaque(c.r), q = opaque(c.q);
+    const int y = w >> 2, I0 = (w & 1) * H, J0 = ((w >> 1) & 1) * H;
+    double* Yo = Y + y * MP * LY;
+    const int lane_off = (q * MP + r) * 8;
+    d4 acc[H][H];
+    // round A: A = Ci^T (Ci read as stored), B = Q^T or P
+    {
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+#pragma unroll
+            for (int j = 0; j < H; ++j) acc[i][j] = zero4();
+        const __amdgpu_buffer_rsrc_t rB = buf_rsrc(y == 0 ? QT : P, MP * MP);
+        const int vB = opaque(lane_off + 16 * J0 * 8);
+        sync();    // the workspace operands and this LDS area are free
+        quad_mma<NT>(buf_rsrc(Cie, MP * MP), opaque(lane_off + 16 * I0 * 8), 1.0,
+                     [&](int kb, int j) { return buf_ld(rB, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+        const double* E = y == 0 ? P : QT;       // P[a][col] (= P[col][a]) or QT[a][col] (= Q[col][a])
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+#pragma unroll
+            for (int j = 0; j < H; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int a = 16 * (I0 + i) + q + 4 * e, col = 16 * (J0 + j) + r;
+                    const double d = a < m ? lam_s[a] : 0.0;
+                    const double ev = E[a * MP + col];
+                    const double v = acc[i][j][e];
+                    Yo[a * LY + col] = y == 0 ? ev * d - v : v + ev * d;
+                }
+    }
+    sync();
+    // round B: T3^T = P Yr^T + Q Yi^T, T4^T = P Yi^T - Q Yr^T (B operands from LDS)
+    {
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+#pragma unroll
+            for (int j = 0; j < H; ++j) acc[i][j] = zero4();
+        const double* Y0 = Y + (y == 0 ? 0 : MP * LY) + q * LY + 16 * J0 + r;    // P term
+        const double* Y1 = Y + (y == 0 ? MP * LY : 0) + q * LY + 16 * J0 + r;    // Q term
+        const int vA = opaque(lane_off + 16 * I0 * 8);
+        quad_mma<NT>(buf_rsrc(P, MP * MP), vA, 1.0, [&](int kb, int j) { return Y0[4 * kb * LY + 16 * j]; }, acc);
+        quad_mma<NT>(buf_rsrc(QT, MP * MP), vA, y == 0 ? 1.0 : -1.0,
+                     [&](int kb, int j) { return Y1[4 * kb * LY + 16 * j]; }, acc);
+    }
+    sync();    // every wave has read Yr^T, Yi^T
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+#pragma unroll
+        for (int j = 0; j < H; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                Yo[(16 * (I0 + i) + q + 4 * e) * LY + 16 * (J0 + j) + r] = acc[i][j][e];
+    sync();
+    // Sr = (T3 + T3^T) / 2, Si = (T4 - T4^T) / 2, G = J - Si, with T3[i][j] = T3^T[j][i]
+    const double* T3t = Y;
+    const double* T4t = Y + MP * LY;
+    // (an opaque thread index keeps the element addresses from being hoisted out of the
+    // Newton loop and kept live)
+    const int t0 = opaque(c.tid);
+    double jv[K::PERE];
+#pragma unroll
+    for (int s = 0; s < K::PERE; ++s) jv[s] = Jse[t0 + s * DT];
+#pragma unroll
+    for (int s = 0; s < K::PERE; ++s) {
+        const int e = t0 + s * DT, i = e / MP, j = e - (e / MP) * MP;
+        Sr[e] = (T3t[j * LY + i] + T3t[i * LY + j]) / 2;
+        const double si = (T4t[j * LY + i] - T4t[i * LY + j]) / 2;
+        Si[e] = si;
+        const double g = jv[s] - si;
+        G[e] = g;
+        gg = fma(g, g, gg);
+    }
+}
+
 // ---------------------------------------------------------------- the engine
 // Body of target b's workgroup (DT threads, dynamic LDS Cfg<NT>::LDS_END doubles).
 template <int NT>
@@ -1153,7 +1296,8 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
     double* Dd = slot(S_D);   double* R = slot(S_R);     double* Z0 = slot(S_Z0);
     double* Pp = slot(S_PP);  double* Hp = slot(S_HP);   double* U = slot(S_U);
     double* Vv = slot(S_VV);  double* T1 = slot(S_T1);   double* T2 = slot(S_T2);
-    double* T3 = slot(S_T3);  double* T4 = slot(S_T4);   double* T5 = slot(S_T5);
+    double* T3 = slot(S_T3);  double* T4 = slot(S_T4);  double* T5 = slot(S_T5);
+    double* QT = slot(S_QT);
     double* lam_s = lds + Cf::OFF_LAM;
 
     const double* Cib = a.Ci + (long long)b * a.sC_b;
@@ -1183,6 +1327,7 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
         W[e] = 0.0;
         P[e] = (i == j && i < m) ? 1.0 : 0.0;
         Q[e] = 0.0;
+        QT[e] = 0.0;
     });
     sync();
     t_setup = wall_clock64() - t_start;
@@ -1205,8 +1350,15 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
                 G[e] = g;
                 gg = fma(g, g, gg);
             });
+        } else if constexpr (ChainCfg<NT>::OK) {
+            gemm<NT, false, false>(c, lds, W, P, [&](int i, int j, double v) {
+                Q[i * MP + j] = -v; QT[j * MP + i] = -v;
+            });
+            sandwich_quad<NT>(c, lds, Cie, P, QT, lam_s, m, Jse, Sr, Si, G, gg);
         } else {
-            gemm<NT, false, false>(c, lds, W, P, [&](int i, int j, double v) { Q[i * MP + j] = -v; });
+            gemm<NT, false, false>(c, lds, W, P, [&](int i, int j, double v) {
+                Q[i * MP + j] = -v; QT[j * MP + i] = -v;
+            });
             // Yr = P diag(lam) - Q Ci -> T1 ; Yi = P Ci + Q diag(lam) -> T2 (hvp_fused.cu:39-80)
             gemm<NT, false, false>(c, lds, Q, Cie, [&](int i, int j, double v) {
                 const double d = j < m ? lam_s[j] : 0.0;
@@ -1248,7 +1400,7 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
             const unsigned long long t0 = wall_clock64();
             int how;
             const int nh = cg_chained<NT, false>(c, lds, [&](int i, int j) { return -G[i * MP + j]; },
-                                                 Sr, Si, P, Q, lam_s, tol, Dd, Z0, how);
+                                                 Sr, Si, P, QT, lam_s, tol, Dd, Z0, how);
             n_hvp += nh; n_cg += nh;
             t_cg += wall_clock64() - t0;
         } else {
@@ -1496,7 +1648,7 @@ void adjoint_kernel(AdjArgs a)
     if constexpr (ChainCfg<NT>::OK) {
         int how;
         const int nh = cg_chained<NT, true>(c, lds, [&](int i, int j) { return ((T1[i * MP + j] - T1[j * MP + i]) / 2) / scale; },
-                                            Sr, Si, Pe, Qe, lam_s, a.tol, X, nullptr, how);
+                                            Sr, Si, Pe, slot(S_QT), lam_s, a.tol, X, nullptr, how);
         it = how == 0 ? nh - 1 : nh;
         status = how == 1 ? 0 : 1;
     }
