@@ -594,7 +594,7 @@ __device__ __forceinline__ int bidx(int I, int J) { return I * (I + 1) / 2 + J; 
 // root (hardware estimate plus two Newton steps) instead of a square root and a divide,
 // and the inverse is right-looking, so the dependent chains stay short. Returns false
 // on a non-positive pivot.
-__device__ bool diag_factor_invert(double* B, int lane) {
+__device__ __forceinline__ bool diag_factor_invert(double* B, int lane) {
     double row[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) row[k] = (lane < 16) ? B[lane * 16 + k] : 0.0;
@@ -1267,6 +1267,82 @@ __device__ __forceinline__ void sandwich_quad(const Ctx& c, double* lds, const d
     }
 }
 
+// ---------------------------------------------------------------- line-search tail in quadrants
+// The exact change of the Armijo test (hvp_line_partial / hvp_line_finish, hvp_fused.cu:240-367):
+// tail = <P' delta, delta Sr + trial (delta Si)>, from delta (T3, exactly skew), trial (T1,
+// exactly skew), P' (T2, exactly symmetric) in global memory. Two rounds of quadrant products:
+//   round 1: waves 0-3: V = delta Si (quadrant w) -> LDS;  waves 4-7: U = delta Sr (registers)
+//   round 2: waves 0-3: T4 = P' delta (registers);  waves 4-7: S = U + trial V -> LDS
+// then waves 0-3 sum T4 * S over their quadrant. A fragments of the skew operands are read
+// transposed (coalesced) with the sign carried by the B operand, so every product equals the
+// untransposed one bit for bit. Returns this thread's partial sum.
+template <int NT>
+__device__ __forceinline__ double ls_tail_quad(const Ctx& c, double* lds, const double* T1, const double* T2,
+                                               const double* T3, const double* Sr, const double* Si) {
+    using K = QuadCfg<NT>;
+    constexpr int MP = K::MP, H = K::H, LY = K::LY;
+    double* Vl = lds + K::OFF;            // V, then S in the second buffer
+    double* Sl = Vl + MP * LY;
+    const int w = c.wave, r = opaque(c.r), q = opaque(c.q);
+    const int quad = w & 3, I0 = (quad & 1) * H, J0 = ((quad >> 1) & 1) * H;
+    const int lane_off = (q * MP + r) * 8;
+    const int vA = opaque(lane_off + 16 * I0 * 8);
+    const int vB = opaque(lane_off + 16 * J0 * 8);
+    const __amdgpu_buffer_rsrc_t rD = buf_rsrc(T3, MP * MP);
+    d4 acc[H][H], acc2[H][H];
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+#pragma unroll
+        for (int j = 0; j < H; ++j) { acc[i][j] = zero4(); acc2[i][j] = zero4(); }
+    sync();    // the LDS blocks are consumed, T1..T3 are written
+    {
+        const __amdgpu_buffer_rsrc_t rS = buf_rsrc(w < 4 ? Si : Sr, MP * MP);
+        quad_mma<NT>(rD, vA, -1.0, [&](int kb, int j) { return buf_ld(rS, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+    }
+    if (w < 4) {
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+#pragma unroll
+            for (int j = 0; j < H; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    Vl[(16 * (I0 + i) + q + 4 * e) * LY + 16 * (J0 + j) + r] = acc[i][j][e];
+    }
+    sync();
+    if (w < 4) {
+        // T4 = P' delta
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+#pragma unroll
+            for (int j = 0; j < H; ++j) acc[i][j] = zero4();
+        quad_mma<NT>(buf_rsrc(T2, MP * MP), vA, 1.0,
+                     [&](int kb, int j) { return buf_ld(rD, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+    } else {
+        // S = U + trial V
+        const double* Vb = Vl + q * LY + 16 * J0 + r;
+        quad_mma<NT>(buf_rsrc(T1, MP * MP), vA, -1.0, [&](int kb, int j) { return Vb[4 * kb * LY + 16 * j]; }, acc2);
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+#pragma unroll
+            for (int j = 0; j < H; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    Sl[(16 * (I0 + i) + q + 4 * e) * LY + 16 * (J0 + j) + r] = acc[i][j][e] + acc2[i][j][e];
+    }
+    sync();
+    double tail = 0.0;
+    if (w < 4) {
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+#pragma unroll
+            for (int j = 0; j < H; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    tail = fma(acc[i][j][e], Sl[(16 * (I0 + i) + q + 4 * e) * LY + 16 * (J0 + j) + r], tail);
+    }
+    return tail;
+}
+
 // ---------------------------------------------------------------- the engine
 // Body of target b's workgroup (DT threads, dynamic LDS Cfg<NT>::LDS_END doubles).
 template <int NT>
@@ -1487,13 +1563,17 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
             const bool valid = cholesky_inverse<NT>(c, lds);
             if (valid) {
                 inverse_product<NT>(c, lds, T2);                        // P' = (L L^T)^-1
-                gemm<NT, false, false>(c, lds, T3, Sr, [&](int i, int j, double v) { U[i * MP + j] = v; });
-                gemm<NT, false, false>(c, lds, T3, Si, [&](int i, int j, double v) { Vv[i * MP + j] = v; });
-                gemm<NT, false, false>(c, lds, T2, T3, [&](int i, int j, double v) { T4[i * MP + j] = v; });
                 double tail = 0.0;
-                gemm<NT, false, false>(c, lds, T1, Vv, [&](int i, int j, double v) {
-                    tail = fma(T4[i * MP + j], U[i * MP + j] + v, tail);   // V (delta Sr + trial delta Si)
-                });
+                if constexpr (ChainCfg<NT>::OK) {
+                    tail = ls_tail_quad<NT>(c, lds, T1, T2, T3, Sr, Si);
+                } else {
+                    gemm<NT, false, false>(c, lds, T3, Sr, [&](int i, int j, double v) { U[i * MP + j] = v; });
+                    gemm<NT, false, false>(c, lds, T3, Si, [&](int i, int j, double v) { Vv[i * MP + j] = v; });
+                    gemm<NT, false, false>(c, lds, T2, T3, [&](int i, int j, double v) { T4[i * MP + j] = v; });
+                    gemm<NT, false, false>(c, lds, T1, Vv, [&](int i, int j, double v) {
+                        tail = fma(T4[i * MP + j], U[i * MP + j] + v, tail);   // V (delta Sr + trial delta Si)
+                    });
+                }
                 wg_sum2(c, tail, first);
                 const double change = first + tail;
                 if (change <= 1e-4 * step * slope) {
