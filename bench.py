@@ -225,14 +225,19 @@ def main():
         log(f"uneven 128: eager {eager_ms:.2f} ms, graph {graph_ms:.2f} ms")
         del Tu, solver, solver_g
 
-    # reference protocol boundary: CPU tensor in -> CPU tensors out (PCIe included)
-    ssn.solve(T_cpu)
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    ssn.solve(T_cpu)
-    torch.cuda.synchronize()
-    pcie_ms = (time.perf_counter() - t3) * 1e3
-    log(f"CPU->CPU batched solve: {pcie_ms:.2f} ms")
+    # reference protocol boundary: CPU tensor in -> CPU tensors out (PCIe included), with the
+    # chunked transfer pipeline (default for this size) and without it
+    def host_solve_ms(pipeline):
+        ssn.solve(T_cpu, pipeline=pipeline)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        for _ in range(2):
+            ssn.solve(T_cpu, pipeline=pipeline)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t3) * 1e3 / 2
+    pcie_ms = host_solve_ms(True)
+    pcie_plain_ms = host_solve_ms(False)
+    log(f"CPU->CPU batched solve: {pcie_ms:.2f} ms pipelined, {pcie_plain_ms:.2f} ms unpipelined")
 
     fd = dual_flops(m, counters)
     f_eigh = 9.0 * m ** 3 * BATCH               # dense symmetric eigensolver with vectors
@@ -288,6 +293,7 @@ def main():
         "plain_solves_per_s": BATCH / (plain_ms * 1e-3) if plain_ms else None,
         "single_call_solves_per_s": BATCH / (fused_ms * 1e-3),
         "pcie_inclusive_solves_per_s": BATCH / (pcie_ms * 1e-3),
+        "pcie_inclusive_unpipelined_solves_per_s": BATCH / (pcie_plain_ms * 1e-3),
         "fwd_bwd_ms": fb_ms,
         "fwd_bwd_solves_per_s": BATCH / (fb_ms * 1e-3),
         "bwd_ms": bwd_ms,

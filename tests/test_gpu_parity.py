@@ -176,3 +176,23 @@ def test_bad_inputs(ssn):
         ssn.solve(torch.zeros(1, 7, 4, dtype=torch.float64))
     with pytest.raises(ValueError):
         ssn.solve(torch.zeros(1, 400, 130, dtype=torch.float64))
+
+
+@pytest.mark.parametrize("case,count", [("n2048_k16", 5), ("n4096_k48", 12)])
+def test_host_pipeline_matches_device(ssn, case, count):
+    """CPU in -> CPU out through the chunked transfer pipeline: every target's solve equals the
+    device-resident solve of the same batch to rounding (the Gram's row split follows the
+    chunk size), with the same iteration counts, and the outputs are CPU tensors."""
+    T = inputs.make_case(case, count)
+    say("pipeline", case, count)
+    Xh, omh, ith = ssn.solve(T, pipeline=True)
+    assert not Xh.is_cuda and not omh.is_cuda and not ith.is_cuda
+    assert Xh.shape == T.shape and Xh.is_contiguous()
+    Xd, omd, itd = ssn.solve(T.cuda())
+    assert torch.equal(itd.cpu(), ith)
+    for j in range(count):
+        assert rel(Xh[j].numpy(), Xd[j].cpu().numpy()) <= 1e-13, j
+        assert rel(omh[j].numpy(), omd[j].cpu().numpy()) <= 1e-12, j
+    # the unpipelined host path gives the device path's results bit for bit
+    Xu, omu, itu = ssn.solve(T, pipeline=False)
+    assert torch.equal(Xu, Xd.cpu()) and torch.equal(omu, omd.cpu()) and torch.equal(itu, itd.cpu())
