@@ -307,6 +307,32 @@ __device__ __forceinline__ void each(const Ctx& c, F f) {
     for (int e = opaque(c.tid); e < MP * MP; e += DT) f(e / MP, e % MP, e);
 }
 
+
+// Element pass over this thread's elements e = tid + s DT (the order of `each`) that issues the
+// loads of a group of elements before any of the group's stores: ld(e, v) loads into v,
+// op(i, j, e, v) computes and stores. The opaque thread index keeps the element addresses
+// inside the solver loops.
+template <int NT, int NV, class LD, class OP>
+__device__ __forceinline__ void each_loaded(const Ctx& c, LD ld, OP op) {
+    constexpr int MP = Cfg<NT>::MP, PE = (MP * MP + DT - 1) / DT;
+    constexpr int GB = PE < 6 ? PE : 6;              // elements per group of loads
+    const int t0 = opaque(c.tid);
+#pragma unroll
+    for (int g = 0; g < PE; g += GB) {
+        double v[GB][NV];
+#pragma unroll
+        for (int u = 0; u < GB; ++u) {
+            const int e = t0 + (g + u) * DT;
+            if (g + u < PE && (PE * DT == MP * MP || e < MP * MP)) ld(e, v[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < GB; ++u) {
+            const int e = t0 + (g + u) * DT;
+            if (g + u < PE && (PE * DT == MP * MP || e < MP * MP)) op(e / MP, e % MP, e, v[u]);
+        }
+    }
+}
+
 // 1 / sqrt(d): hardware estimate refined by two Newton steps (a few ulp).
 __device__ __forceinline__ double rsqrt_refined(double d) {
     double y = __builtin_amdgcn_rsq(d);
@@ -1063,12 +1089,15 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
                 for (int e = 0; e < 4; ++e) M[(16 * I + q + 4 * e) * LS + 16 * J + r] += accZ[I][e];
         }
         sync();
-        // H = Z - Z^T on the owned elements; curvature <p, Hp>
+        // H = Z - Z^T on the owned elements; curvature <p, Hp>. The element indices are
+        // recomputed from an opaque copy of the thread index every iteration rather than
+        // kept live across the products.
+        const int ti = opaque(tid);
         double H[UPT];
         double curv = 0.0;
 #pragma unroll
         for (int s = 0; s < UPT; ++s) {
-            const int u = tid + s * DT;
+            const int u = ti + s * DT;
             H[s] = 0.0;
             if (u < UN) {
                 int i, j;
@@ -1084,7 +1113,7 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
         double rr = 0.0, rzn = 0.0;
 #pragma unroll
         for (int s = 0; s < UPT; ++s) {
-            const int u = tid + s * DT;
+            const int u = ti + s * DT;
             if (u < UN) {
                 int i, j;
                 upper_ij<MP>(u, i, j);
@@ -1103,7 +1132,7 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
         const double beta = rzn / rz;
 #pragma unroll
         for (int s = 0; s < UPT; ++s) {
-            const int u = tid + s * DT;
+            const int u = ti + s * DT;
             if (u < UN) {
                 int i, j;
                 upper_ij<MP>(u, i, j);
@@ -1528,7 +1557,8 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
         }
         // -- fallback to the preconditioned gradient (:163-164)
         double slope = 0.0, dummy2 = 0.0;
-        each<NT>(c, [&](int, int, int e) { slope = fma(G[e], Dd[e], slope); });
+        each_loaded<NT, 2>(c, [&](int e, double (&v)[2]) { v[0] = G[e]; v[1] = Dd[e]; },
+                           [&](int, int, int, const double (&v)[2]) { slope = fma(v[0], v[1], slope); });
         wg_sum2(c, slope, dummy2);
         if (slope >= 0.0) {
             double s2 = 0.0, d3 = 0.0;
@@ -1545,12 +1575,15 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
         for (int bt = 0; bt < MAX_BACKTRACK; ++bt) {
             ++n_bt;
             double first = 0.0;
-            each<NT>(c, [&](int i, int j, int e) {
-                const double de = step * Dd[e];
-                const double dt = step * Dd[j * MP + i];
+            // delta = step d; trial = skew(omega + delta) (:168-170). omega and d are exactly
+            // skew, so with a = omega_ij + delta_ij the transposed sum is exactly -a and
+            // (a - (-a)) / 2 = a: the skew part needs no transposed reads and is bitwise the same.
+            each_loaded<NT, 3>(c, [&](int e, double (&v)[3]) { v[0] = Dd[e]; v[1] = W[e]; v[2] = G[e]; },
+                               [&](int, int, int e, const double (&v)[3]) {
+                const double de = step * v[0];
                 T3[e] = de;                                              // delta
-                T1[e] = ((W[e] + de) - (W[j * MP + i] + dt)) / 2;        // trial = skew(omega + delta)
-                first = fma(G[e], de, first);
+                T1[e] = v[1] + de;                                       // trial
+                first = fma(v[2], de, first);
             });
             sync();
             // K = I + trial^2 into the packed lower blocks
@@ -1577,7 +1610,8 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
                 wg_sum2(c, tail, first);
                 const double change = first + tail;
                 if (change <= 1e-4 * step * slope) {
-                    each<NT>(c, [&](int, int, int e) { W[e] = T1[e]; P[e] = T2[e]; });
+                    each_loaded<NT, 2>(c, [&](int e, double (&v)[2]) { v[0] = T1[e]; v[1] = T2[e]; },
+                                       [&](int, int, int e, const double (&v)[2]) { W[e] = v[0]; P[e] = v[1]; });
                     sync();
                     accepted = true;
                     break;
