@@ -81,18 +81,31 @@ __device__ __forceinline__ void stream_gemm_body(const double* __restrict__ T, l
 #pragma unroll
     for (int j = 0; j < NJ; ++j) acc[j] = zero4();
 
-#pragma unroll 2
-    for (int ks = 0; ks < KS; ++ks) {
-        const int k = 4 * ks + q;
-        const int part = k / MP, kk = k - part * MP;     // 0 a, 1 d, 2 gt, 3 gb
-        double a = 0.0;
-        if (row_ok && kk < m) {
-            const double* src = part == 0 ? Arow : part == 1 ? Drow : part == 2 ? Gtrow : Gbrow;
-            a = src[kk];
-        }
-        const double* Fk = F + (long long)ks * NJ * 64 + lane;
+    // k-step ks reads column 4 ks + q of part (4 ks) / MP of the row (MP is a multiple of 4, so
+    // the part is the same for every lane); the row elements are requested DA k-steps ahead
+    // (they stream from HBM) and the fragments of M one k-step ahead (L2)
+    auto aload = [&](int ks) -> double {
+        const int part = (4 * ks) / MP, kk = 4 * ks - part * MP + q;
+        const double* src = part == 0 ? Arow : part == 1 ? Drow : part == 2 ? Gtrow : Gbrow;
+        return (row_ok && kk < m) ? src[kk] : 0.0;
+    };
+    constexpr int DA = 8;
+    double ar[DA], br[2][NJ];
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) acc[j] = mfma(a, Fk[j * 64], acc[j]);
+    for (int d = 0; d < DA; ++d) ar[d] = aload(d);
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) br[0][j] = F[j * 64 + lane];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        const double a = ar[ks % DA];
+        if (ks + DA < KS) ar[ks % DA] = aload(ks + DA);
+        if (ks + 1 < KS) {
+            const double* Fn = F + (long long)(ks + 1) * NJ * 64 + lane;
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) br[(ks + 1) & 1][j] = Fn[j * 64];
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc[j] = mfma(a, br[ks & 1][j], acc[j]);
     }
 
     // lane holds D[q + 4i][r] of each tile: rows wave*16 + q + 4i, column 16 J + r
