@@ -225,6 +225,17 @@ def solve(T, return_stats=False, split=False, pipeline=None):
 PIPELINE_CHUNKS = 4
 PIPELINE_STREAMS = 4
 _pipe_streams = {}
+_pipe_staging = {}
+
+
+def _pinned_staging(numel):
+    """Pinned input staging of the pipeline, kept for the process (grown when too small)."""
+    buf = _pipe_staging.get("buf")
+    if buf is None or buf.numel() < numel:
+        _pipe_staging["buf"] = None
+        buf = torch.empty(numel, dtype=torch.float64, pin_memory=True)
+        _pipe_staging["buf"] = buf
+    return buf[:numel]
 
 
 def _pipeline_streams(dev, count):
@@ -254,7 +265,7 @@ def _solve_host_pipelined(T, return_stats, split, chunks=None, streams=None):
     bounds = [(batch * c // nch, batch * (c + 1) // nch) for c in range(nch)]
     h2d, d2h, comp = _pipeline_streams(dev, min(streams or PIPELINE_STREAMS, nch))
     cur = torch.cuda.current_stream(dev)
-    staging = torch.empty(T.shape, dtype=T.dtype, pin_memory=True)
+    staging = _pinned_staging(T.numel()).view(T.shape)
     Xh = torch.empty((batch, 2 * n, m), dtype=torch.float64, pin_memory=True)
     Td = torch.empty(T.shape, dtype=T.dtype, device=dev)
     Xd = torch.empty((batch, 2 * n, m), dtype=torch.float64, device=dev)
