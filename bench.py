@@ -270,6 +270,13 @@ def main():
                      roof_hbm("ssn::split_recover_kernel<6,5> (+ split_recovery_fragments)", b_rec, f_rec,
                               stages_split["recover"])]
     dominant = max(kernels, key=lambda k: k["avg_ms"])
+    # the dual engine and the eigensolver run one workgroup (one CU) per target: also state the
+    # fraction of the FP64 MFMA peak of the CUs the launch occupies
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    for k in kernels:
+        if "dual_kernel" in k["kernel"] or "eigh_kernel" in k["kernel"]:
+            k["occupied_cus"] = min(BATCH, n_cu)
+            k["frac_of_occupied_cus"] = k["achieved"] / (FP64_PEAK_TFLOPS * min(BATCH, n_cu) / n_cu)
 
     out = {
         "metric": METRIC,

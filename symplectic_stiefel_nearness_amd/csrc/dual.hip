@@ -532,6 +532,24 @@ __device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, in
                 const double* Ao = cur ? A1 : A0;
                 double* An = cur ? A0 : A1;
                 const double2* cso = reinterpret_cast<const double2*>(cur ? R1 : R0);
+                // V in global memory: the eigenvector rows of this round are requested before
+                // the block work, so their latency overlaps it (the columns depend on the round
+                // only; the rotation is applied after the blocks)
+                double gvv[V_IN_LDS ? 1 : NVB * JV][2];
+                int gva[V_IN_LDS ? 1 : NVB * JV];
+                int gvp = 0, gvq = 0;
+                if constexpr (!V_IN_LDS) {
+                    if (v_on) {
+                        const int s_ = opaque(vS), g_ = opaque(vG);
+                        gvp = player(s_, round, m); gvq = player(m - 1 - s_, round, m);
+#pragma unroll
+                        for (int k = 0; k < NVB * JV; ++k) {
+                            const int i = g_ + k * vgroups;
+                            gva[k] = i < m ? (int)__umul24(i, LV) : vdummy;
+                            gvv[k][0] = V[gva[k] + gvp]; gvv[k][1] = V[gva[k] + gvq];
+                        }
+                    }
+                }
                 // Addresses are formed (and made opaque) while gathering; the stores use
                 // them as plain registers: an asm statement between stores would force a
                 // wait for every outstanding LDS operation.
@@ -560,7 +578,14 @@ __device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, in
                         An[wa[u][0]] = z00; An[wa[u][1]] = z01; An[wa[u][2]] = z10; An[wa[u][3]] = z11;
                     }
                 }
-                if (v_on) {
+                if (!V_IN_LDS && v_on) {
+                    const double2 rv = cso[opaque(vS)];
+#pragma unroll
+                    for (int k = 0; k < NVB * JV; ++k) {
+                        V[gva[k] + gvp] = fma(rv.x, gvv[k][0], -(rv.y * gvv[k][1]));
+                        V[gva[k] + gvq] = fma(rv.y, gvv[k][0], rv.x * gvv[k][1]);
+                    }
+                } else if (v_on) {
                     const int s_ = opaque(vS), g_ = opaque(vG);   // row offsets stay in the loop
                     const int vp = player(s_, round, m), vq = player(m - 1 - s_, round, m);
                     const double2 rv = cso[s_];
