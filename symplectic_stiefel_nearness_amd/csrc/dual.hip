@@ -30,6 +30,15 @@
 // element-wise updates mirror the reference's explicitly rounded (non-contracted) ops
 #pragma clang fp contract(off)
 
+// Tuning of the chained CG products: k-blocks of A fragments in flight in the second product,
+// and whether each k-block is fenced from the scheduler (keeps the loads from being hoisted).
+#ifndef SSN_CG_PREFETCH
+#define SSN_CG_PREFETCH 3
+#endif
+#ifndef SSN_CG_SCHED_BARRIER
+#define SSN_CG_SCHED_BARRIER 1
+#endif
+
 namespace ssn {
 
 constexpr int DW = 8;                 // waves per target
@@ -1021,7 +1030,7 @@ __device__ __forceinline__ void chain_term(int t, int J, int r, int q, const dou
     constexpr int is = 16 * 8;                           // A-fragment step per row block
     // the sign of -Q is carried by the B operand ((-Q) V = Q (-V) product by product), so the
     // prefetched A fragments are used as loaded
-    constexpr int D = 3;
+    constexpr int D = SSN_CG_PREFETCH;
     double an[D][NT];
 #pragma unroll
     for (int d = 0; d < D; ++d)
@@ -1037,7 +1046,7 @@ __device__ __forceinline__ void chain_term(int t, int J, int r, int q, const dou
 #pragma unroll
             for (int I = 0; I < NT; ++I) an[kb % D][I] = buf_ld(rA, vA, (kb + D) * ks + I * is);
         }
-        __builtin_amdgcn_sched_barrier(0);
+        if (SSN_CG_SCHED_BARRIER) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
