@@ -15,6 +15,8 @@ int gram_chunk_count(int batch, int n);
 hipError_t launch_gram_partial(hipStream_t, const double*, long long, long long, int, int, int, int, double*);
 hipError_t launch_gram_reduce(hipStream_t, const double*, int, int, int, double*, double*, long long, int);
 hipError_t launch_recovery_fragments(hipStream_t, const double*, const double*, int, int, double*);
+hipError_t launch_recover_pq(hipStream_t, const double*, long long, long long, const double*, const double*, int,
+                             int, int, double*, long long, long long, size_t);
 hipError_t launch_recover(hipStream_t, const double*, long long, long long, const double*, int, int, int,
                           double*, long long, long long);
 size_t dual_workspace_doubles(int batch, int m);
@@ -216,11 +218,14 @@ int ssn_recover(void* stream, const double* T, long long sT_b, long long sT_r, c
 {
     if (!shape_ok(batch, n, m) || !T || !P || !Q || !X) return SSN_ERR_ARGS;
     if (batch == 0) return SSN_OK;
-    int rc = current_device_ready(nullptr);
+    size_t lds = 0;
+    int rc = current_device_ready(&lds);
     if (rc) return rc;
     const size_t need = (size_t)batch * 4 * pad16(m) * pad16(m) * sizeof(double);
     if (!ws || ws_bytes < need) return SSN_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
+    const hipError_t e = launch_recover_pq(s, T, sT_b, sT_r, P, Q, batch, n, m, X, sX_b, sX_r, lds);
+    if (e != hipErrorNotSupported) return hip_status(e);
     rc = hip_status(launch_recovery_fragments(s, P, Q, batch, m, (double*)ws));
     if (rc) return rc;
     return hip_status(launch_recover(s, T, sT_b, sT_r, (const double*)ws, batch, n, m, X, sX_b, sX_r));
@@ -265,6 +270,8 @@ static int solve_impl(bool split, void* stream, const double* T, long long sT_b,
     if (split)
         return hip_status(launch_split_recover(s, T, sT_b, sT_r, w + L.pf, w + L.qf, w + L.frag, batch, n, m,
                                                X, sX_b, sX_r));
+    const hipError_t e = launch_recover_pq(s, T, sT_b, sT_r, w + L.pf, w + L.qf, batch, n, m, X, sX_b, sX_r, lds);
+    if (e != hipErrorNotSupported) return hip_status(e);
     rc = hip_status(launch_recovery_fragments(s, w + L.pf, w + L.qf, batch, m, w + L.frag));
     if (rc) return rc;
     return hip_status(launch_recover(s, T, sT_b, sT_r, w + L.frag, batch, n, m, X, sX_b, sX_r));

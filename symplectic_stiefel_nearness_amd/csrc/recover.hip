@@ -220,6 +220,129 @@ hipError_t launch_backward_stream(hipStream_t stream, const double* T, long long
     return launch_stream<2>(stream, T, sT_b, sT_r, G, sG_b, sG_r, frag, batch, n, m, dT, sD_b, sD_r);
 }
 
+
+// ---------------------------------------------------------------- recovery with P, Q in LDS
+// The same product X = [a | d] M, M = [[P^T, -Q^T], [Q^T, P^T]], for mp <= 96: each workgroup
+// holds its target's P and Q in LDS (2 mp (mp + 1) doubles, odd row stride) and streams row
+// tiles of that target, so the B fragments M[4 ks + q][16 J + r] = +-(P or Q)[16 J' + r][k'] are
+// LDS reads instead of L2 reads of a pre-built fragment array (no fragment pass either). The
+// products and their order are those of stream_gemm_kernel, so X is bitwise the same.
+constexpr int RL_THREADS = 512;
+constexpr int RL_WAVES = RL_THREADS / WAVE;
+
+template <int NT>
+__host__ __device__ constexpr int recover_lds_doubles() { return 2 * (16 * NT) * (16 * NT + 1); }
+
+template <int NT>
+__global__ __launch_bounds__(RL_THREADS)
+void recover_lds_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
+                        const double* __restrict__ P, const double* __restrict__ Q, int n, int m,
+                        double* __restrict__ X, long long sX_b, long long sX_r, int wpt)
+{
+    constexpr int MP = 16 * NT, LS = MP + 1, NJ = 2 * NT, KS = 2 * MP / 4;
+    extern __shared__ double sh[];
+    double* Ps = sh;
+    double* Qs = sh + MP * LS;
+    const int b = blockIdx.y;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, r = lane & 15, q = lane >> 4;
+    const double* Pb = P + (long long)b * MP * MP;
+    const double* Qb = Q + (long long)b * MP * MP;
+    for (int e = tid; e < MP * MP; e += RL_THREADS) {
+        const int i = e / MP, j = e - (e / MP) * MP;
+        Ps[i * LS + j] = Pb[e];
+        Qs[i * LS + j] = Qb[e];
+    }
+    __syncthreads();
+    const double* Tb = T + (long long)b * sT_b;
+    double* Xb = X + (long long)b * sX_b;
+    const int ntiles = (n + 15) / 16;
+    for (int tile = blockIdx.x * RL_WAVES + wave; tile < ntiles; tile += wpt * RL_WAVES) {
+        const int row = tile * 16 + r;
+        const bool row_ok = row < n;
+        const double* Arow = Tb + (long long)row * sT_r;
+        const double* Drow = Tb + (long long)(row + n) * sT_r;
+        auto aload = [&](int ks) -> double {
+            const int part = (4 * ks) / MP, kk = 4 * ks - part * MP + q;
+            return (row_ok && kk < m) ? (part ? Drow : Arow)[kk] : 0.0;
+        };
+        d4 acc[NJ];
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc[j] = zero4();
+        constexpr int DA = 8;
+        double ar[DA];
+#pragma unroll
+        for (int d = 0; d < DA; ++d) ar[d] = aload(d);
+        const double* Pl = Ps + r * LS + q;
+        const double* Ql = Qs + r * LS + q;
+        // B fragment J of k-step ks: M[k][j] = [P^T | -Q^T] (k < mp) or [Q^T | P^T] (k >= mp),
+        // i.e. +-(P or Q)[16 J' + r][k'], read from LDS one k-step ahead
+        auto bload = [&](int ks, double (&dst)[NJ]) {
+            const int part = (4 * ks) / MP, kk0 = 4 * ks - part * MP;
+#pragma unroll
+            for (int J = 0; J < NJ; ++J) {
+                const int jc = J / NT, Jm = J - jc * NT;
+                const double v = ((part == jc) ? Pl : Ql)[16 * Jm * LS + kk0];
+                dst[J] = (part == 0 && jc == 1) ? -v : v;
+            }
+        };
+        double br[2][NJ];
+        bload(0, br[0]);
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const double a = ar[ks % DA];
+            if (ks + DA < KS) ar[ks % DA] = aload(ks + DA);
+            if (ks + 1 < KS) bload(ks + 1, br[(ks + 1) & 1]);
+#pragma unroll
+            for (int J = 0; J < NJ; ++J) acc[J] = mfma(a, br[ks & 1][J], acc[J]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const bool top = j < NT;
+            const int col = (top ? j : j - NT) * 16 + r;
+            if (col >= m) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int orow = tile * 16 + q + 4 * i;
+                if (orow < n)
+                    Xb[(long long)(top ? orow : orow + n) * sX_r + col] = acc[j][i];
+            }
+        }
+    }
+}
+
+// Recovery straight from P and Q (no fragment array) when the LDS holds both: mp <= 96.
+// Returns hipErrorNotSupported otherwise (the caller builds fragments and uses launch_recover).
+hipError_t launch_recover_pq(hipStream_t stream, const double* T, long long sT_b, long long sT_r,
+                             const double* P, const double* Q, int batch, int n, int m,
+                             double* X, long long sX_b, long long sX_r, size_t lds_limit)
+{
+    const int nt = pad16(m) / 16;
+    if (nt > 6) return hipErrorNotSupported;
+    const int ntiles = (n + 15) / 16;
+    int wpt = (256 + batch - 1) / batch;                      // about one workgroup per CU
+    const int most = (ntiles + RL_WAVES - 1) / RL_WAVES;
+    if (wpt > most) wpt = most;
+    if (wpt < 1) wpt = 1;
+    dim3 grid(wpt, batch), block(RL_THREADS);
+#define SSN_RL_CASE(NTV) \
+    case NTV: { \
+        const size_t lds = (size_t)recover_lds_doubles<NTV>() * 8; \
+        if (lds > lds_limit) return hipErrorNotSupported; \
+        hipError_t err = hipFuncSetAttribute((const void*)recover_lds_kernel<NTV>, \
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        if (err != hipSuccess) return err; \
+        hipLaunchKernelGGL(recover_lds_kernel<NTV>, grid, block, lds, stream, T, sT_b, sT_r, P, Q, n, m, \
+                           X, sX_b, sX_r, wpt); \
+        break; }
+    switch (nt) {
+        SSN_RL_CASE(1) SSN_RL_CASE(2) SSN_RL_CASE(3) SSN_RL_CASE(4) SSN_RL_CASE(5) SSN_RL_CASE(6)
+        default: return hipErrorNotSupported;
+    }
+#undef SSN_RL_CASE
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- uneven batches
 __global__ void recovery_fragments_uneven_kernel(const UTarget* __restrict__ tg)
 {
