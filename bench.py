@@ -261,11 +261,11 @@ def main():
                 "algorithmic_bytes_per_launch": nbytes,
                 "fp64_equivalent_tflops": flops / (ms * 1e-3) / 1e12, "avg_ms": ms}
 
-    kernels = [roof_mfma("ssn::gram_partial_kernel<6> (+ gram_reduce_kernel)", f_gram, gram_ms),
+    kernels = [roof_mfma("ssn::gram_partial_kernel<6> (+ gram_reduce_tiles_kernel)", f_gram, gram_ms),
                roof_mfma("ssn::eigh_kernel<3,true>", f_eigh, eigh_ms),
                roof_mfma("ssn::dual_kernel<6>", fd, newton_ms),
                roof_mfma("ssn::stream_gemm_kernel<6,1> (+ build_recovery_fragments)", f_rec, rec_ms)]
-    kernels_split = [roof_hbm("ssn::split_gram_kernel<6,2,5> (+ gram_reduce_kernel)", b_gram, f_gram,
+    kernels_split = [roof_hbm("ssn::split_gram_kernel<6,2,5> (+ gram_reduce_tiles_kernel)", b_gram, f_gram,
                               stages_split["gram"]),
                      roof_hbm("ssn::split_recover_kernel<6,5> (+ split_recovery_fragments)", b_rec, f_rec,
                               stages_split["recover"])]

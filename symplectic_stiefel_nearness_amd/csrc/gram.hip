@@ -182,15 +182,65 @@ __device__ __forceinline__ void gram_reduce_element(const double* __restrict__ p
     Cib[(long long)i * ldc + j] = (i == j) ? 0.0 : (i < j ? ci_upper : -ci_upper);
 }
 
-__global__ void gram_reduce_kernel(const double* __restrict__ partial, int chunks, int m, int mp,
-                                   double* __restrict__ Cr, double* __restrict__ Ci,
-                                   long long sC_b, int ldc)
+// Same sums as gram_reduce_element, one workgroup of 256 threads per (tile pair I <= J, target):
+// thread (r, c) sums element (r, c) of the upper tile (I, J) of slot 0 and of tiles (I, J),
+// (J, I) of slot 1 over the chunks in ascending order, so every chunk read is a contiguous
+// 16-double row; the transposed operands are then exchanged through LDS. Each output element
+// is the same expression of the same sums as in gram_reduce_element (bitwise equal results).
+__global__ __launch_bounds__(256)
+void gram_reduce_tiles_kernel(const double* __restrict__ partial, int chunks, int m, int mp,
+                              double* __restrict__ Cr, double* __restrict__ Ci,
+                              long long sC_b, int ldc)
 {
+    constexpr int LS = 17;
+    __shared__ double l0[16 * LS], l1[16 * LS], l2[16 * LS];
     const int b = blockIdx.y;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= m * m) return;
-    gram_reduce_element(partial + (long long)b * chunks * 2 * mp * mp, chunks, m, mp,
-                        Cr + (long long)b * sC_b, Ci + (long long)b * sC_b, ldc, e);
+    int I = 0, rem = blockIdx.x;                 // blockIdx.x -> (I, J), I <= J, row-major upper
+    const int nt = mp / 16;
+    while (rem >= nt - I) { rem -= nt - I; ++I; }
+    const int J = I + rem;
+    const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
+    const double* p = partial + (long long)b * chunks * 2 * mp * mp;
+    const int oij = (I * 16 + r) * mp + J * 16 + c;     // element (r, c) of tile (I, J)
+    const int oji = (J * 16 + r) * mp + I * 16 + c;     // element (r, c) of tile (J, I)
+    double cr = 0.0, sij = 0.0, sji = 0.0;
+    for (int k = 0; k < chunks; ++k) {
+        const double* pc = p + (long long)k * 2 * mp * mp;
+        cr += pc[oij];
+        sij += pc[mp * mp + oij];
+        sji += pc[mp * mp + oji];
+    }
+    l0[r * LS + c] = cr;
+    l1[r * LS + c] = sij;
+    l2[r * LS + c] = sji;
+    __syncthreads();
+    double* Crb = Cr + (long long)b * sC_b;
+    double* Cib = Ci + (long long)b * sC_b;
+    {   // tile (I, J): i = 16I + r, j = 16J + c
+        const int i = I * 16 + r, j = J * 16 + c;
+        double vr, vi;
+        if (I < J) {                              // i < j
+            vr = cr;
+            vi = sij - l2[c * LS + r];
+        } else if (r <= c) {                      // diagonal tile, upper part (lo = i)
+            vr = cr;
+            vi = (r == c) ? 0.0 : sij - l1[c * LS + r];
+        } else {                                  // diagonal tile, lower part (lo = j)
+            vr = l0[c * LS + r];
+            vi = -(l1[c * LS + r] - sij);
+        }
+        if (i < m && j < m) {
+            Crb[(long long)i * ldc + j] = vr;
+            Cib[(long long)i * ldc + j] = vi;
+        }
+    }
+    if (I < J) {   // tile (J, I): i = 16J + r > j = 16I + c, lo = j
+        const int i = J * 16 + r, j = I * 16 + c;
+        if (i < m && j < m) {
+            Crb[(long long)i * ldc + j] = l0[c * LS + r];
+            Cib[(long long)i * ldc + j] = -(l1[c * LS + r] - sji);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- uneven batches
@@ -410,9 +460,9 @@ hipError_t launch_cross_gram(hipStream_t stream, const double* T, long long sT_b
 hipError_t launch_gram_reduce(hipStream_t stream, const double* partial, int batch, int chunks, int m,
                               double* Cr, double* Ci, long long sC_b, int ldc)
 {
-    const int mp = pad16(m);
-    dim3 grid((m * m + 255) / 256, batch), block(256);
-    hipLaunchKernelGGL(gram_reduce_kernel, grid, block, 0, stream, partial, chunks, m, mp,
+    const int mp = pad16(m), nt = mp / 16;
+    dim3 grid(nt * (nt + 1) / 2, batch), block(256);
+    hipLaunchKernelGGL(gram_reduce_tiles_kernel, grid, block, 0, stream, partial, chunks, m, mp,
                        Cr, Ci, sC_b, ldc);
     return hipGetLastError();
 }
