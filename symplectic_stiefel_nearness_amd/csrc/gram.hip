@@ -14,6 +14,8 @@
 #include "common.h"
 #include "uneven.h"
 
+#include <type_traits>
+
 namespace ssn {
 
 constexpr int GRAM_WAVES = 8;
@@ -26,46 +28,77 @@ struct GramShape {
     static constexpr int NCR = NT * (NT + 1) / 2;   // upper tiles of Cr
     static constexpr int NAD = NT * NT;             // all tiles of A^T D
     static constexpr int NTOT = NCR + NAD;
-    static constexpr int MAXT = (NTOT + GRAM_WAVES - 1) / GRAM_WAVES;
     static constexpr int LDS_STRIDE = MP + 2;       // doubles per staged row
     static constexpr int STAGE = 2 * GRAM_ROWS * LDS_STRIDE;   // A rows then D rows
     static constexpr int PER_THREAD = (2 * GRAM_ROWS * MP + GRAM_THREADS - 1) / GRAM_THREADS;
 };
 
-__device__ __forceinline__ void tile_of(int t, int nt, int ncr, int& I, int& J, bool& is_cr) {
+// Tile t of the plan: t < NCR is the upper tile (I <= J) of A^T A + D^T D in column-major
+// upper order, the others are the tiles of A^T D in row-major order.
+struct GramTile { int I, J; bool cr; };
+
+constexpr GramTile gram_tile(int t, int nt) {
+    const int ncr = nt * (nt + 1) / 2;
     if (t < ncr) {
-        is_cr = true;
         int col = 0, rem = t;
         while (rem > col) { rem -= col + 1; ++col; }
-        I = rem; J = col;              // I <= J
-    } else {
-        is_cr = false;
-        const int u = t - ncr;
-        I = u / nt; J = u % nt;
+        return GramTile{rem, col, true};
+    }
+    return GramTile{(t - ncr) / nt, (t - ncr) % nt, false};
+}
+
+// Compile-time assignment of the tiles to the waves of a workgroup: tiles in plan order, each to
+// the wave with the fewest MFMAs so far (an A^T A + D^T D tile costs two per k-step, an A^T D
+// tile one), so the busiest wave issues ceil(MFMAs / 8) per k-step.
+template <int NT>
+struct GramPlan {
+    static constexpr int MAXW = 24;
+    struct Table { int tile[GRAM_WAVES][MAXW]; int count[GRAM_WAVES]; int maxt; };
+    static constexpr Table make() {
+        Table tb{};
+        int load[GRAM_WAVES] = {};
+        for (int t = 0; t < GramShape<NT>::NTOT; ++t) {
+            int w = 0;
+            for (int v = 1; v < GRAM_WAVES; ++v)
+                if (load[v] < load[w]) w = v;
+            tb.tile[w][tb.count[w]++] = t;
+            load[w] += gram_tile(t, NT).cr ? 2 : 1;
+        }
+        for (int v = 0; v < GRAM_WAVES; ++v)
+            if (tb.count[v] > tb.maxt) tb.maxt = tb.count[v];
+        return tb;
+    }
+    static constexpr Table table = make();
+    static_assert(table.maxt <= MAXW, "tile plan overflow");
+};
+
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
     }
 }
 
 // partial layout: [batch][chunks][2][MP][MP]; slot 0 = A^T A + D^T D (upper tiles),
 // slot 1 = A^T D (all tiles).
-// Body of one (row chunk, target) workgroup; `stage` is 2 * GramShape<NT>::STAGE doubles of LDS.
-template <int NT>
-__device__ __forceinline__ void gram_partial_body(const double* __restrict__ T, long long sT_b, long long sT_r,
-                                                  int n, int m, int chunk_rows, int chunks,
-                                                  double* __restrict__ partial, int chunk, int b,
+// The chunk loop of wave W: every thread stages rows, the wave accumulates the tiles of its
+// plan from the 2 NT column fragments of each k-step, read once from LDS.
+template <int NT, int W>
+__device__ __forceinline__ void gram_partial_wave(const double* __restrict__ Tb, long long sT_r, int n, int m,
+                                                  int row_begin, int row_end, double* __restrict__ out,
                                                   double* __restrict__ stage_lds)
 {
     using S = GramShape<NT>;
     constexpr int MP = S::MP;
+    constexpr int NW = GramPlan<NT>::table.count[W];
 
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int r = lane & 15, q = lane >> 4;
-    const int row_begin = chunk * chunk_rows;
-    const int row_end = min(n, row_begin + chunk_rows);
-    const double* Tb = T + (long long)b * sT_b;
 
-    d4 acc[S::MAXT];
+    d4 acc[NW > 0 ? NW : 1];
 #pragma unroll
-    for (int s = 0; s < S::MAXT; ++s) acc[s] = zero4();
+    for (int s = 0; s < NW; ++s) acc[s] = zero4();
 
     // Static per-thread load slots: element e of a stage -> (half, row, col).
     double pre[S::PER_THREAD];
@@ -98,15 +131,6 @@ __device__ __forceinline__ void gram_partial_body(const double* __restrict__ T, 
         }
     };
 
-    // tile coordinates of this wave, fixed for the whole chunk
-    int tI[S::MAXT], tJ[S::MAXT];
-    bool tcr[S::MAXT];
-#pragma unroll
-    for (int s = 0; s < S::MAXT; ++s) {
-        const int t = wave + s * GRAM_WAVES;
-        tile_of(t < S::NTOT ? t : 0, NT, S::NCR, tI[s], tJ[s], tcr[s]);
-    }
-
     // one barrier per stage: the buffer written at stage st+1 is the other one, and
     // the barrier of stage st+1 orders every read of stage st before the write at st+2
     const int nstages = (row_end - row_begin + GRAM_ROWS - 1) / GRAM_ROWS;
@@ -121,39 +145,53 @@ __device__ __forceinline__ void gram_partial_body(const double* __restrict__ T, 
 #pragma unroll
         for (int kq = 0; kq < GRAM_ROWS / 4; ++kq) {
             const int row = (kq * 4 + q) * S::LDS_STRIDE + r;
+            double fa[NT], fd[NT];      // fragments this wave does not use are never read
 #pragma unroll
-            for (int s = 0; s < S::MAXT; ++s) {
-                if (wave + s * GRAM_WAVES < S::NTOT) {
-                    const int I = tI[s] * 16, J = tJ[s] * 16;
-                    const double aI = sA[row + I];
-                    const double dJ = sD[row + J];
-                    if (tcr[s]) {
-                        const double aJ = sA[row + J];
-                        const double dI = sD[row + I];
-                        acc[s] = mfma(aI, aJ, acc[s]);
-                        acc[s] = mfma(dI, dJ, acc[s]);
-                    } else {
-                        acc[s] = mfma(aI, dJ, acc[s]);
-                    }
+            for (int j = 0; j < NT; ++j) { fa[j] = sA[row + 16 * j]; fd[j] = sD[row + 16 * j]; }
+            static_for<0, NW>([&](auto si) {
+                constexpr int s = decltype(si)::value;
+                constexpr GramTile g = gram_tile(GramPlan<NT>::table.tile[W][s], NT);
+                if constexpr (g.cr) {
+                    acc[s] = mfma(fa[g.I], fa[g.J], acc[s]);
+                    acc[s] = mfma(fd[g.I], fd[g.J], acc[s]);
+                } else {
+                    acc[s] = mfma(fa[g.I], fd[g.J], acc[s]);
                 }
-            }
+            });
         }
     }
 
-    double* out = partial + ((long long)b * chunks + chunk) * 2 * MP * MP;
+    static_for<0, NW>([&](auto si) {
+        constexpr int s = decltype(si)::value;
+        constexpr GramTile g = gram_tile(GramPlan<NT>::table.tile[W][s], NT);
+        double* o = out + (g.cr ? 0 : MP * MP);
 #pragma unroll
-    for (int s = 0; s < S::MAXT; ++s) {
-        if (wave + s * GRAM_WAVES < S::NTOT) {
-            double* o = out + (tcr[s] ? 0 : MP * MP);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                o[(tI[s] * 16 + q + 4 * i) * MP + tJ[s] * 16 + r] = acc[s][i];
-        }
+        for (int i = 0; i < 4; ++i)
+            o[(g.I * 16 + q + 4 * i) * MP + g.J * 16 + r] = acc[s][i];
+    });
+}
+
+// Body of one (row chunk, target) workgroup; `stage` is 2 * GramShape<NT>::STAGE doubles of LDS.
+template <int NT>
+__device__ __forceinline__ void gram_partial_body(const double* __restrict__ T, long long sT_b, long long sT_r,
+                                                  int n, int m, int chunk_rows, int chunks,
+                                                  double* __restrict__ partial, int chunk, int b,
+                                                  double* __restrict__ stage_lds)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int row_begin = chunk * chunk_rows;
+    const int row_end = min(n, row_begin + chunk_rows);
+    const double* Tb = T + (long long)b * sT_b;
+    double* out = partial + ((long long)b * chunks + chunk) * 2 * GramShape<NT>::MP * GramShape<NT>::MP;
+    switch (wave) {
+#define SSN_GW(W) case W: gram_partial_wave<NT, W>(Tb, sT_r, n, m, row_begin, row_end, out, stage_lds); break;
+        SSN_GW(0) SSN_GW(1) SSN_GW(2) SSN_GW(3) SSN_GW(4) SSN_GW(5) SSN_GW(6) SSN_GW(7)
+#undef SSN_GW
     }
 }
 
 template <int NT>
-__global__ __launch_bounds__(GRAM_THREADS)
+__global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(NT <= 6 ? 4 : 1)))   // 2 WGs per CU
 void gram_partial_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
                          int n, int m, int chunk_rows, int chunks,
                          double* __restrict__ partial)
@@ -246,7 +284,7 @@ void gram_reduce_tiles_kernel(const double* __restrict__ partial, int chunks, in
 // ---------------------------------------------------------------- uneven batches
 // Persistent grid over the (target, chunk) items of one order of a plan (largest first).
 template <int NT>
-__global__ __launch_bounds__(GRAM_THREADS)
+__global__ __launch_bounds__(GRAM_THREADS) __attribute__((amdgpu_waves_per_eu(NT <= 6 ? 4 : 1)))
 void gram_uneven_kernel(const UTarget* __restrict__ tg, const UItem* __restrict__ items, int nitems)
 {
     __shared__ double stage[2 * GramShape<NT>::STAGE];
@@ -402,9 +440,13 @@ __global__ void cross_gram_reduce_kernel(const double* __restrict__ partial, int
 // ------------------------------------------------------------------ launchers
 namespace ssn {
 
+#ifndef SSN_GRAM_TARGET_WGS
+#define SSN_GRAM_TARGET_WGS 512
+#endif
+
 int gram_chunk_count(int batch, int n) {
     // enough workgroups to cover the chip about twice; at least 256 rows per chunk
-    const int target_wgs = 512;
+    const int target_wgs = SSN_GRAM_TARGET_WGS;
     int chunks = (target_wgs + batch - 1) / batch;
     const int max_chunks = (n + 255) / 256;
     if (chunks > max_chunks) chunks = max_chunks;
