@@ -319,26 +319,27 @@ struct CrossShape {
     static constexpr int PER_THREAD = (4 * GRAM_ROWS * MP + GRAM_THREADS - 1) / GRAM_THREADS;
 };
 
-template <int NT>
-__global__ __launch_bounds__(GRAM_THREADS)
-void cross_gram_partial_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
-                               const double* __restrict__ G, long long sG_b, long long sG_r,
-                               int n, int m, int chunk_rows, int chunks, double* __restrict__ partial)
+// Wave W of a cross Gram workgroup owns the units [9W, 9W + 9) of the 2 NT^2 units (tile t,
+// part p) in row-major tile order (p = 0: Mr, p = 1: Mi; two MFMAs per unit per k-step), so
+// every wave issues the same number of MFMAs and reads each column fragment it uses once per
+// k-step. Per unit the products are accumulated in the same order as a whole-tile owner would.
+template <int NT, int W>
+__device__ __forceinline__ void cross_gram_wave(const double* __restrict__ Tb, long long sT_r,
+                                                const double* __restrict__ Gb, long long sG_r,
+                                                int n, int m, int row_begin, int row_end,
+                                                double* __restrict__ out, double* __restrict__ stage_lds)
 {
     using S = CrossShape<NT>;
     constexpr int MP = S::MP;
-    __shared__ double stage[2][S::STAGE];
-    const int chunk = blockIdx.x, b = blockIdx.y;
-    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    constexpr int NU = 2 * NT * NT;
+    constexpr int U0 = NU * W / GRAM_WAVES, U1 = NU * (W + 1) / GRAM_WAVES;
+    constexpr int NW = U1 - U0;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int r = lane & 15, q = lane >> 4;
-    const int row_begin = chunk * chunk_rows;
-    const int row_end = min(n, row_begin + chunk_rows);
-    const double* Tb = T + (long long)b * sT_b;
-    const double* Gb = G + (long long)b * sG_b;
 
-    d4 accr[S::MAXT], acci[S::MAXT];
+    d4 acc[NW > 0 ? NW : 1];
 #pragma unroll
-    for (int s = 0; s < S::MAXT; ++s) { accr[s] = zero4(); acci[s] = zero4(); }
+    for (int s = 0; s < NW; ++s) acc[s] = zero4();
 
     double pre[S::PER_THREAD];
     auto load_stage = [&](int r0) {
@@ -374,7 +375,7 @@ void cross_gram_partial_kernel(const double* __restrict__ T, long long sT_b, lon
     const int nstages = (row_end - row_begin + GRAM_ROWS - 1) / GRAM_ROWS;
     if (nstages > 0) load_stage(row_begin);
     for (int st = 0; st < nstages; ++st) {
-        double* cur = stage[st & 1];
+        double* cur = stage_lds + (st & 1) * S::STAGE;
         store_stage(cur);
         __syncthreads();
         if (st + 1 < nstages) load_stage(row_begin + (st + 1) * GRAM_ROWS);
@@ -382,38 +383,58 @@ void cross_gram_partial_kernel(const double* __restrict__ T, long long sT_b, lon
         const double* sD = cur + GRAM_ROWS * S::LDS_STRIDE;
         const double* sGt = cur + 2 * GRAM_ROWS * S::LDS_STRIDE;
         const double* sGb = cur + 3 * GRAM_ROWS * S::LDS_STRIDE;
-#pragma unroll 1
-        for (int kq = 0; kq < GRAM_ROWS / 4; ++kq) {
-            int row = (kq * 4 + q) * S::LDS_STRIDE + r;
-            asm volatile("" : "+v"(row));
 #pragma unroll
-            for (int s = 0; s < S::MAXT; ++s) {
-                const int t = wave + s * GRAM_WAVES;
-                if (t < S::NTOT) {
-                    const int I = (t / NT) * 16, J = (t % NT) * 16;
-                    const double aI = sA[row + I], dI = sD[row + I];
-                    const double gt = sGt[row + J], gb = sGb[row + J];
-                    accr[s] = mfma(aI, gt, accr[s]);
-                    accr[s] = mfma(dI, gb, accr[s]);
-                    acci[s] = mfma(aI, gb, acci[s]);
-                    acci[s] = mfma(-dI, gt, acci[s]);
-                }
+        for (int kq = 0; kq < GRAM_ROWS / 4; ++kq) {
+            const int row = (kq * 4 + q) * S::LDS_STRIDE + r;
+            double fa[NT], fd[NT], ft[NT], fb[NT];   // fragments this wave does not use are never read
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                fa[j] = sA[row + 16 * j]; fd[j] = sD[row + 16 * j];
+                ft[j] = sGt[row + 16 * j]; fb[j] = sGb[row + 16 * j];
             }
+            static_for<0, NW>([&](auto si) {
+                constexpr int s = decltype(si)::value;
+                constexpr int u = U0 + s, I = (u / 2) / NT, J = (u / 2) % NT;
+                if constexpr (u % 2 == 0) {
+                    acc[s] = mfma(fa[I], ft[J], acc[s]);
+                    acc[s] = mfma(fd[I], fb[J], acc[s]);
+                } else {
+                    acc[s] = mfma(fa[I], fb[J], acc[s]);
+                    acc[s] = mfma(-fd[I], ft[J], acc[s]);
+                }
+            });
         }
     }
+    static_for<0, NW>([&](auto si) {
+        constexpr int s = decltype(si)::value;
+        constexpr int u = U0 + s, I = (u / 2) / NT, J = (u / 2) % NT;
+        double* o = out + (u % 2) * MP * MP;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            o[(I * 16 + q + 4 * i) * MP + J * 16 + r] = acc[s][i];
+    });
+}
+
+template <int NT>
+__global__ __launch_bounds__(GRAM_THREADS)
+void cross_gram_partial_kernel(const double* __restrict__ T, long long sT_b, long long sT_r,
+                               const double* __restrict__ G, long long sG_b, long long sG_r,
+                               int n, int m, int chunk_rows, int chunks, double* __restrict__ partial)
+{
+    using S = CrossShape<NT>;
+    constexpr int MP = S::MP;
+    __shared__ double stage[2][S::STAGE];
+    const int chunk = blockIdx.x, b = blockIdx.y;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int row_begin = chunk * chunk_rows;
+    const int row_end = min(n, row_begin + chunk_rows);
+    const double* Tb = T + (long long)b * sT_b;
+    const double* Gb = G + (long long)b * sG_b;
     double* out = partial + ((long long)b * chunks + chunk) * 2 * MP * MP;
-#pragma unroll
-    for (int s = 0; s < S::MAXT; ++s) {
-        const int t = wave + s * GRAM_WAVES;
-        if (t < S::NTOT) {
-            const int I = t / NT, J = t % NT;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int o = (I * 16 + q + 4 * i) * MP + J * 16 + r;
-                out[o] = accr[s][i];
-                out[MP * MP + o] = acci[s][i];
-            }
-        }
+    switch (wave) {
+#define SSN_CW(W) case W: cross_gram_wave<NT, W>(Tb, sT_r, Gb, sG_r, n, m, row_begin, row_end, out, &stage[0][0]); break;
+        SSN_CW(0) SSN_CW(1) SSN_CW(2) SSN_CW(3) SSN_CW(4) SSN_CW(5) SSN_CW(6) SSN_CW(7)
+#undef SSN_CW
     }
 }
 
