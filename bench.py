@@ -242,7 +242,9 @@ def main():
 
     fd = dual_flops(m, counters)
     f_eigh = 9.0 * m ** 3 * BATCH               # dense symmetric eigensolver with vectors
-    f_gram = 8.0 * N * m * m * BATCH            # full Cr, Ci products (FP64-equivalent)
+    # Gram: Cr = [A; D]^T [A; D] is symmetric (2n m (m + 1) flops for its upper triangle), and
+    # Ci = A^T D - (A^T D)^T needs the one product A^T D (2 n m^2)
+    f_gram = (2.0 * N * m * (m + 1) + 2.0 * N * m * m) * BATCH
     f_rec = 8.0 * N * m * m * BATCH
     b_gram = 16.0 * N * m * BATCH               # read T once
     b_rec = 32.0 * N * m * BATCH                # read T, write X
@@ -264,7 +266,7 @@ def main():
     kernels = [roof_mfma("ssn::gram_partial_kernel<6> (+ gram_reduce_tiles_kernel)", f_gram, gram_ms),
                roof_mfma("ssn::eigh_kernel<3,true>", f_eigh, eigh_ms),
                roof_mfma("ssn::dual_kernel<6>", fd, newton_ms),
-               roof_mfma("ssn::stream_gemm_kernel<6,1> (+ build_recovery_fragments)", f_rec, rec_ms)]
+               roof_mfma("ssn::recover_lds_kernel<6>", f_rec, rec_ms)]
     kernels_split = [roof_hbm("ssn::split_gram_kernel<6,2,5> (+ gram_reduce_tiles_kernel)", b_gram, f_gram,
                               stages_split["gram"]),
                      roof_hbm("ssn::split_recover_kernel<6,5> (+ split_recovery_fragments)", b_rec, f_rec,
