@@ -141,9 +141,28 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for _ in range(3):
-        ssn.solve_device(T, check=False)
+        X_call = ssn.solve_device(T, check=False)[0]
     torch.cuda.synchronize()
     fused_ms = (time.perf_counter() - t1) * 1e3 / 3
+
+    # Batches in flight (ssn.solve_batches): consecutive 64-target solves go round-robin over the
+    # current stream and nb - 1 side streams, so the per-target kernels of one batch (one workgroup
+    # per target, 64 of the 256 CUs) run beside those of the next ones. Not `value`: this is the
+    # rate of a caller that hands over several batches at once.
+    flight = {}
+    fl_count = 12
+    for nb in (2, 4):
+        ssn.solve_batches([T] * (2 * nb), in_flight=nb)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        fl_out = ssn.solve_batches([T] * fl_count, in_flight=nb)
+        torch.cuda.synchronize()
+        fms = (time.perf_counter() - t1) * 1e3 / fl_count
+        fok = all(torch.equal(o[0], X_call) for o in fl_out)
+        del fl_out
+        flight[str(nb)] = {"ms_per_batch": fms, "solves_per_s": BATCH / (fms * 1e-3), "results_equal": fok}
+        log(f"{nb} batches in flight: {fms:.3f} ms per 64-target batch, results equal {fok}")
+    del X_call
 
     # plain: per-target loop of the FP64 solver (experiment.py:75-80), device-resident
     plain_ms = None
@@ -302,6 +321,7 @@ def main():
         "fwd_bwd_ms_split_forward": fb[True],
         "plain_solves_per_s": BATCH / (plain_ms * 1e-3) if plain_ms else None,
         "single_call_solves_per_s": BATCH / (fused_ms * 1e-3),
+        "batches_in_flight": flight,
         "pcie_inclusive_solves_per_s": BATCH / (pcie_ms * 1e-3),
         "pcie_inclusive_unpipelined_solves_per_s": BATCH / (pcie_plain_ms * 1e-3),
         "fwd_bwd_ms": fb_ms,

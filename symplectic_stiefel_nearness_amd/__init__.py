@@ -9,12 +9,12 @@ Drop-in for the reference package `symplectic_projection` (src/symplectic_projec
 The arithmetic runs in libssn.so (hand-written HIP for gfx950) through a C ABI
 (include/ssn.h); this package is the host-side mirror of the reference interface.
 """
-from .dual import (initialize_kernels, solve, solve_device, project, backward_device, gram_chunks,
+from .dual import (initialize_kernels, solve, solve_device, solve_batches, project, backward_device, gram_chunks,
                    dual_batch, dual_batch_staged, recover_matmul, FEAS_TOL, MAX_NEWTON, MAX_CG,
                    MAX_BACKTRACK)
 
 from .uneven import UnevenSolver, solve_uneven
 from .autotune import autotune
 
-__all__ = ["initialize_kernels", "solve", "solve_device", "project", "backward_device", "gram_chunks",
+__all__ = ["initialize_kernels", "solve", "solve_device", "solve_batches", "project", "backward_device", "gram_chunks",
            "dual_batch", "dual_batch_staged", "recover_matmul", "UnevenSolver", "solve_uneven", "autotune"]

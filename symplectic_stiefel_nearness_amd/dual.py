@@ -187,6 +187,57 @@ def solve_device(T, return_stats=False, check=True, split=False):
     return X, omega, iters
 
 
+MAX_IN_FLIGHT = 4                      # hardware queues of a process (GPU_MAX_HW_QUEUES default)
+_flight = {}
+
+
+def _flight_streams(dev, count):
+    """Side streams for batches in flight, created once per device (default priority)."""
+    key = dev.index
+    have = _flight.setdefault(key, [])
+    while len(have) < count:
+        have.append(torch.cuda.Stream(dev))
+    return have[:count]
+
+
+@torch.no_grad()
+def solve_batches(Ts, in_flight=MAX_IN_FLIGHT, split=False, return_stats=False):
+    """Device solves of several batches, up to `in_flight` of them running at once.
+
+    Each batch is solved exactly as solve_device(T) solves it (same results, bit for bit); the
+    batches go round-robin over the current stream and in_flight - 1 side streams, so the
+    per-target launches of one batch (one workgroup per target) share the GPU with those of
+    the next ones. Outputs are usable on the current stream when this returns; the status of
+    every batch is checked at the end (one synchronisation). Returns a list of
+    (X, omega, iterations[, stats]) in the order of Ts.
+    """
+    _require_ready()
+    assert 1 <= in_flight <= MAX_IN_FLIGHT, f"in_flight must be in 1..{MAX_IN_FLIGHT}"
+    Ts = list(Ts)
+    if not Ts:
+        return []
+    dev = Ts[0].device
+    assert all(T.is_cuda and T.device == dev for T in Ts), "solve_batches expects device tensors on one device"
+    cur = torch.cuda.current_stream(dev)
+    streams = [cur] + _flight_streams(dev, in_flight - 1)
+    for s_ in streams[1:]:
+        s_.wait_stream(cur)                            # inputs are ready
+    res = []
+    for i, T in enumerate(Ts):
+        s_ = streams[i % len(streams)]
+        with torch.cuda.stream(s_):
+            X, omega, iters, _, stats = _run_solve(T, split)
+        if s_ is not cur:
+            for t in (X, omega, iters, stats["status"], stats["counters"]):
+                t.record_stream(cur)                   # freed by the caller after use on cur
+        res.append((X, omega, iters, stats))
+    for s_ in streams[1:]:
+        cur.wait_stream(s_)
+    for r in res:
+        _raise_for_status(r[3]["status"])
+    return [r if return_stats else r[:3] for r in res]
+
+
 @torch.no_grad()
 def solve(T, return_stats=False, split=False, pipeline=None):
     """(X, omega, iterations) for float64 targets T [batch, 2n, 2k] (dual_recommended_hvp.py:231-242).

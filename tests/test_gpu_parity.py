@@ -196,3 +196,20 @@ def test_host_pipeline_matches_device(ssn, case, count):
     # the unpipelined host path gives the device path's results bit for bit
     Xu, omu, itu = ssn.solve(T, pipeline=False)
     assert torch.equal(Xu, Xd.cpu()) and torch.equal(omu, omd.cpu()) and torch.equal(itu, itd.cpu())
+
+
+def test_solve_batches_in_flight(ssn):
+    """Several batches (different shapes) with up to four in flight: each equals solve_device of
+    the same batch bit for bit, and is usable on the current stream when the call returns."""
+    say("solve_batches")
+    Ts = [inputs.make_case("n2048_k16", 6).cuda(), inputs.make_case("n256_k8", 3).cuda(),
+          inputs.make_targets(150, 10, 2, seeds=[21, 22]).cuda(), inputs.make_case("n2048_k16", 6).cuda()[2:],
+          inputs.make_case("n256_k8", 1).cuda()]
+    for in_flight in (4, 2, 1):
+        out = ssn.solve_batches(Ts, in_flight=in_flight, return_stats=True)
+        assert len(out) == len(Ts)
+        for T, (X, om, it, st) in zip(Ts, out):
+            Xd, omd, itd = ssn.solve_device(T)
+            assert torch.equal(X, Xd) and torch.equal(om, omd) and torch.equal(it, itd)
+            assert int(st["status"].max()) == 0
+    assert ssn.solve_batches([]) == []
