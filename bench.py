@@ -61,6 +61,49 @@ def cpu_baseline(budget_s=12.0):
                       f"{solved} solves in {dt:.1f} s, CPU in -> CPU out, no PCIe"}
 
 
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _cpu_worker(budget_s):
+    """One single-thread oracle worker (a spawned process): (solves, seconds)."""
+    from threadpoolctl import threadpool_limits
+    import torch
+    from symplectic_stiefel_nearness_amd import inputs
+    from oracle import dual as odual
+    torch.set_num_threads(1)
+    sample = inputs.make_case(CASE, 2).numpy()
+    solved = 0
+    with threadpool_limits(limits=1):
+        odual.solve(sample[:1])            # warm-up
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s:
+            odual.solve(sample)
+            solved += sample.shape[0]
+        dt = time.perf_counter() - t0
+    return solved, dt
+
+
+def cpu_baseline_all_cores(budget_s=10.0):
+    """The oracle on every core of this process's CPU share (at most 16): one spawned
+    single-thread worker per core, each solving the same 2-target sample for budget_s."""
+    import multiprocessing as mp
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    with mp.get_context("spawn").Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [budget_s] * cores)
+    rate = sum(c / dt for c, dt in res)
+    return {"value": rate, "unit": "solves/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"oracle Dual (NumPy FP64), {cores} processes x 1 BLAS thread, each on the first 2 "
+                      f"seeded {CASE} targets for {budget_s:.0f} s ({sum(c for c, _ in res)} solves); "
+                      "value = sum of the per-process rates, CPU in -> CPU out"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,6 +387,9 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"]["cpu_model"] = cpu_model()
+        log("cpu baseline, all cores ...")
+        out["cpu_baseline_all_cores"] = cpu_baseline_all_cores()
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
