@@ -46,6 +46,13 @@ int ssn_padded_order(int m);
 int ssn_last_hip_error(void);
 const char* ssn_status_string(int status);
 
+/* Host-to-host copy of `bytes` bytes from src to dst on `threads` host threads (clamped to 1..64):
+ * the copy of a CPU target into pinned staging before its host-to-device transfer. Replaces
+ * `host.copy_(T[:, start:stop])` into the pinned buffer of gram_chunks
+ * (dual_recommended_hvp.py:86-90), whose rate otherwise follows the caller's intra-op thread
+ * setting. Host-only (no HIP call). SSN_ERR_ARGS for a null pointer with bytes > 0. */
+int ssn_host_copy(void* dst, const void* src, size_t bytes, int threads);
+
 /* Bytes of device scratch needed by ssn_solve (and sufficient for ssn_gram, ssn_dual,
  * ssn_recover) for `batch` targets of shape 2n x m. 0 for unsupported shapes. */
 size_t ssn_workspace_bytes(int batch, int n, int m);

@@ -44,6 +44,7 @@ SIGNATURES = {
     "ssn_padded_order": (_I, [_I]),
     "ssn_last_hip_error": (_I, []),
     "ssn_status_string": (ctypes.c_char_p, [_I]),
+    "ssn_host_copy": (_I, [_P, _P, _S, _I]),
     "ssn_workspace_bytes": (_S, [_I, _I, _I]),
     "ssn_gram": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _P, _L, _I, _P, _S]),
     "ssn_dual": (_I, [_P, _P, _P, _L, _I, _I, _I, _P, _L, _I, _P, _P, _P, _P, _P, _P, _S]),

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <algorithm>
+#include <cstring>
+#include <thread>
 #include <vector>
 #include "common.h"
 #include "uneven.h"
@@ -119,6 +121,23 @@ extern "C" {
 int ssn_init(void) { return current_device_ready(nullptr); }
 
 const char* ssn_version(void) { return "ssn-gfx950 0.1.0"; }
+
+int ssn_host_copy(void* dst, const void* src, size_t bytes, int threads) {
+    if (bytes == 0) return SSN_OK;
+    if (!dst || !src) return SSN_ERR_ARGS;
+    threads = std::max(1, std::min(threads, 64));
+    const size_t page = 4096;
+    size_t per = (bytes + threads - 1) / threads;
+    per = (per + page - 1) / page * page;                 // page-aligned slices
+    char* d = static_cast<char*>(dst);
+    const char* sp = static_cast<const char*>(src);
+    std::vector<std::thread> pool;
+    for (size_t off = per; off < bytes; off += per)
+        pool.emplace_back([=] { std::memcpy(d + off, sp + off, std::min(per, bytes - off)); });
+    std::memcpy(d, sp, std::min(per, bytes));             // the caller's thread takes the first slice
+    for (auto& t : pool) t.join();
+    return SSN_OK;
+}
 
 int ssn_padded_order(int m) { return pad16(m); }
 

@@ -8,6 +8,7 @@ All arithmetic runs in libssn.so (HIP, gfx950); there is no CPU fallback.
 """
 from __future__ import annotations
 
+import os
 import torch
 
 from . import _lib
@@ -279,6 +280,12 @@ _pipe_streams = {}
 _pipe_staging = {}
 
 
+def _host_copy_threads():
+    """Host threads for the staging copy: the process's CPU share, at most 8 (a 400 MB copy
+    saturates at about 4 threads on the host measured)."""
+    return min(8, len(os.sched_getaffinity(0)))
+
+
 def _pinned_staging(numel):
     """Pinned input staging of the pipeline, kept for the process (grown when too small)."""
     buf = _pipe_staging.get("buf")
@@ -298,12 +305,14 @@ def _pipeline_streams(dev, count):
     return _pipe_streams[key]
 
 
-def _solve_host_pipelined(T, return_stats, split, chunks=None, streams=None):
+def _solve_host_pipelined(T, return_stats, split, chunks=None, streams=None, direct=None):
     """CPU-in / CPU-out solve with the transfers overlapped (dual_recommended_hvp.py:79-96, :194-228).
 
     The reference streams rows through pinned staging buffers (ROW_CHUNK rows) around one
     batched solve. Here the batch is cut into chunks of whole targets: each chunk is copied
-    into pinned staging on the host, sent to the device on a copy stream, solved on one of
+    into pinned staging on the host (ssn_host_copy on up to 8 native threads, so the rate does
+    not follow torch's intra-op thread setting; with fewer than 4 cores the chunk is sent from
+    T's pageable memory instead), sent to the device on a copy stream, solved on one of
     the solve streams as soon as it has arrived, and its X is returned on a second copy stream
     into the pinned output while later chunks are still in flight. All device buffers are
     allocated up front on the caller's stream; every stream is drained before returning.
@@ -324,6 +333,12 @@ def _solve_host_pipelined(T, return_stats, split, chunks=None, streams=None):
     iters = torch.zeros(batch, dtype=torch.int64, device=dev)
     status = torch.zeros(batch, dtype=torch.int32, device=dev)
     counters = torch.zeros((batch, 12), dtype=torch.int32, device=dev)
+    threads = _host_copy_threads()
+    # With fewer than 4 cores a host copy into pinned staging (17 ms for 403 MB on one core) is
+    # slower than the driver's own pageable transfer (14 ms): send the chunks straight from T.
+    if direct is None:
+        direct = threads < 4
+    contiguous = T.is_contiguous()
     biggest = max(b1 - b0 for b0, b1 in bounds)
     wsb = lib.ssn_workspace_bytes(biggest, n, m)
     ws = [_workspace(wsb, dev) for _ in comp]          # one per solve stream, reused in order
@@ -331,10 +346,20 @@ def _solve_host_pipelined(T, return_stats, split, chunks=None, streams=None):
         s_.wait_stream(cur)                            # buffers and inputs are ready
     fn, name = (lib.ssn_solve_split, "ssn_solve_split") if split else (lib.ssn_solve, "ssn_solve")
     for c, (b0, b1) in enumerate(bounds):
-        staging[b0:b1].copy_(T[b0:b1])                 # host copy into pinned staging
-        with torch.cuda.stream(h2d):
-            Td[b0:b1].copy_(staging[b0:b1], non_blocking=True)
-            arrived = h2d.record_event()
+        if direct:
+            with torch.cuda.stream(h2d):
+                Td[b0:b1].copy_(T[b0:b1], non_blocking=True)    # pageable: returns once copied
+                arrived = h2d.record_event()
+        else:
+            if contiguous:                             # host copy into pinned staging, native threads
+                nbytes = (b1 - b0) * T.stride(0) * T.element_size()
+                rc = lib.ssn_host_copy(staging[b0:b1].data_ptr(), T[b0:b1].data_ptr(), nbytes, threads)
+                _lib.check(rc, "ssn_host_copy")
+            else:
+                staging[b0:b1].copy_(T[b0:b1])
+            with torch.cuda.stream(h2d):
+                Td[b0:b1].copy_(staging[b0:b1], non_blocking=True)
+                arrived = h2d.record_event()
         k = c % len(comp)
         s_ = comp[k]
         s_.wait_event(arrived)

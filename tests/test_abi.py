@@ -56,3 +56,17 @@ def test_host_queries(lib):
     assert L.ssn_workspace_bytes(2, 100, 96) < L.ssn_workspace_bytes(4, 100, 96)
     assert L.ssn_workspace_bytes(1, 10, 3) == 0          # odd order: unsupported
     assert L.ssn_workspace_bytes(1, 10, 130) == 0        # above the largest order
+
+
+def test_host_copy(lib):
+    """ssn_host_copy is host-only: exact copies for odd sizes and thread counts, argument errors."""
+    import numpy as np
+    lib = _lib.load()                      # the bound entry points (size_t argument)
+    rng = np.random.default_rng(0)
+    for nbytes, threads in ((1, 1), (4097, 3), (1 << 20, 8), (12345, 64), (100, 200)):
+        src = rng.integers(0, 256, nbytes, dtype=np.uint8)
+        dst = np.zeros_like(src)
+        assert lib.ssn_host_copy(dst.ctypes.data, src.ctypes.data, nbytes, threads) == 0
+        assert np.array_equal(dst, src)
+    assert lib.ssn_host_copy(None, None, 0, 4) == 0
+    assert lib.ssn_host_copy(None, ctypes.c_void_p(1), 8, 4) == -1

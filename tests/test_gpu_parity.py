@@ -213,3 +213,16 @@ def test_solve_batches_in_flight(ssn):
             assert torch.equal(X, Xd) and torch.equal(om, omd) and torch.equal(it, itd)
             assert int(st["status"].max()) == 0
     assert ssn.solve_batches([]) == []
+
+
+def test_host_pipeline_pageable_chunks(ssn):
+    """The pipeline's few-cores route (chunks sent from T's pageable memory, no staging copy) and a
+    non-contiguous CPU input give the same results as the staged route."""
+    T = inputs.make_case("n2048_k16", 6)
+    say("pipeline pageable")
+    Xa, oma, ita = ssn.dual._solve_host_pipelined(T, False, False, direct=True)
+    Xb, omb, itb = ssn.dual._solve_host_pipelined(T, False, False, direct=False)
+    assert torch.equal(Xa, Xb) and torch.equal(oma, omb) and torch.equal(ita, itb)
+    Tn = T.transpose(1, 2).contiguous().transpose(1, 2)          # column-major targets
+    Xc, omc, itc = ssn.dual._solve_host_pipelined(Tn, False, False, direct=False)
+    assert torch.equal(Xc, Xb) and torch.equal(omc, omb) and torch.equal(itc, itb)
