@@ -61,6 +61,22 @@ def cpu_baseline(budget_s=12.0):
                       f"{solved} solves in {dt:.1f} s, CPU in -> CPU out, no PCIe"}
 
 
+PROFILE = ROOT / "profiles" / "solve_fp64_n4096_k48_b64_kernel_stats.md"
+
+
+def profiled_traffic(kernel, grid_size):
+    """Copy-calibrated HBM bytes per launch of `kernel` at `grid_size` threads, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (None if it is not there)."""
+    try:
+        for line in PROFILE.read_text(encoding="utf-8").splitlines():
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 5 and cells[0] == f"`{kernel}`" and cells[1] == str(grid_size):
+                return float(cells[4]) * 1e6
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def cpu_model():
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -333,6 +349,15 @@ def main():
                               stages_split["gram"]),
                      roof_hbm("ssn::split_recover_kernel<6,5> (+ split_recovery_fragments)", b_rec, f_rec,
                               stages_split["recover"])]
+    # HBM traffic per launch of each kernel (the main kernel of a stage) at this workload's grid
+    for k, (pname, grid) in zip(kernels, (("void ssn::gram_partial_kernel<6>", 4096 * BATCH),
+                                          ("void ssn::eigh_kernel<3, true>", 512 * BATCH),
+                                          ("void ssn::dual_kernel<6>", 512 * BATCH),
+                                          ("void ssn::recover_lds_kernel<6>", 2048 * BATCH))):
+        k["traffic"] = profiled_traffic(pname, grid)
+        if k["traffic"] is not None:
+            k["traffic_source"] = (f"{PROFILE.relative_to(ROOT)}: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
+                                   f"(factors from a 256 MiB device copy) of {pname}, bytes per launch")
     dominant = max(kernels, key=lambda k: k["avg_ms"])
     # the dual engine and the eigensolver run one workgroup (one CU) per target: also state the
     # fraction of the FP64 MFMA peak of the CUs the launch occupies
