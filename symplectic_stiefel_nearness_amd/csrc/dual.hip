@@ -1195,6 +1195,60 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
     return nh;
 }
 
+// K = I + trial^2 on the packed lower 16 x 16 blocks, from a copy of trial in LDS (row stride
+// MP + 1, in the K-panel area of the staged products). Wave w forms the lower tiles (I >= J)
+// t = w, w + 8, ... of trial trial, each over k in ascending order of 4-blocks: the same products
+// in the same order as the staged product, so K is bitwise the same; the upper tiles, which
+// the Cholesky factorisation does not read, are not formed.
+template <int NT>
+struct TrialCfg {
+    static constexpr int MP = 16 * NT, LS = MP + 1, NB = NT * (NT + 1) / 2;
+    static constexpr int MAXT = (NB + 7) / 8;
+    static constexpr bool OK = MP * LS <= 2 * Cfg<NT>::PANEL;    // the copy fits the panel area
+};
+
+template <int NT>
+__device__ __forceinline__ void trial_square_lower(const Ctx& c, double* lds) {
+    using TC = TrialCfg<NT>;
+    constexpr int LS = TC::LS, K4 = 4 * NT;
+    const double* Tl = lds + Cfg<NT>::OFF_PANEL;
+    double* blk = lds + Cfg<NT>::OFF_BLK;
+    const int r = opaque(c.r), q = opaque(c.q);
+    int tI[TC::MAXT], tJ[TC::MAXT];
+#pragma unroll
+    for (int s = 0; s < TC::MAXT; ++s) {
+        const int t = c.wave + 8 * s;
+        int I = 0, rem = t < TC::NB ? t : 0;
+        while (rem > I) { rem -= I + 1; ++I; }
+        tI[s] = I; tJ[s] = rem;
+    }
+    d4 acc[TC::MAXT];
+#pragma unroll
+    for (int s = 0; s < TC::MAXT; ++s) acc[s] = zero4();
+#pragma unroll 4
+    for (int kb = 0; kb < K4; ++kb) {
+#pragma unroll
+        for (int s = 0; s < TC::MAXT; ++s) {
+            if (c.wave + 8 * s < TC::NB) {
+                const double a = Tl[(16 * tI[s] + r) * LS + 4 * kb + q];
+                const double b = Tl[(4 * kb + q) * LS + 16 * tJ[s] + r];
+                acc[s] = mfma(a, b, acc[s]);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < TC::MAXT; ++s) {
+        if (c.wave + 8 * s < TC::NB) {
+            double* o = blk + bidx(tI[s], tJ[s]) * 256;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = q + 4 * e;       // row and column within the block
+                o[i * 16 + r] = ((tI[s] == tJ[s] && i == r) ? 1.0 : 0.0) + acc[s][e];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- sandwich in quadrants
 // S = N C N^H in the eigenbasis (dual_recommended_hvp.py:120-129, hvp_sandwich_front
 // hvp_fused.cu:39-80 and the m x m recovery), N = P + iQ, C = diag(lambda) + i Ci, as two
@@ -1612,20 +1666,27 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
             // delta = step d; trial = skew(omega + delta) (:168-170). omega and d are exactly
             // skew, so with a = omega_ij + delta_ij the transposed sum is exactly -a and
             // (a - (-a)) / 2 = a: the skew part needs no transposed reads and is bitwise the same.
+            double* Tl = lds + Cf::OFF_PANEL;                          // trial copy (row stride MP + 1)
             each_loaded<NT, 3>(c, [&](int e, double (&v)[3]) { v[0] = Dd[e]; v[1] = W[e]; v[2] = G[e]; },
-                               [&](int, int, int e, const double (&v)[3]) {
+                               [&](int i, int j, int e, const double (&v)[3]) {
                 const double de = step * v[0];
+                const double tr = v[1] + de;
                 T3[e] = de;                                              // delta
-                T1[e] = v[1] + de;                                       // trial
+                T1[e] = tr;                                              // trial
+                if constexpr (TrialCfg<NT>::OK) Tl[i * (MP + 1) + j] = tr;
                 first = fma(v[2], de, first);
             });
             sync();
             // K = I + trial^2 into the packed lower blocks
-            double* blk = lds + Cf::OFF_BLK;
-            gemm<NT, false, false>(c, lds, T1, T1, [&](int i, int j, double v) {
-                const int I = i >> 4, J = j >> 4;
-                if (J <= I) blk[bidx(I, J) * 256 + (i & 15) * 16 + (j & 15)] = (i == j ? 1.0 : 0.0) + v;
-            });
+            if constexpr (TrialCfg<NT>::OK) {
+                trial_square_lower<NT>(c, lds);
+            } else {
+                double* blk = lds + Cf::OFF_BLK;
+                gemm<NT, false, false>(c, lds, T1, T1, [&](int i, int j, double v) {
+                    const int I = i >> 4, J = j >> 4;
+                    if (J <= I) blk[bidx(I, J) * 256 + (i & 15) * 16 + (j & 15)] = (i == j ? 1.0 : 0.0) + v;
+                });
+            }
             sync();
             const bool valid = cholesky_inverse<NT>(c, lds);
             if (valid) {
