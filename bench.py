@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-reps", type=int, default=1)
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-in-flight", action="store_true",
+                    help="skip the batches-in-flight measurement (its launches overlap in a kernel trace)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with one 256 MiB device-to-device copy (scale for HBM byte counters)")
     args = ap.parse_args()
@@ -210,7 +212,7 @@ def main():
     # rate of a caller that hands over several batches at once.
     flight = {}
     fl_count = 12
-    for nb in (2, 4):
+    for nb in (() if args.no_in_flight else (2, 4)):
         ssn.solve_batches([T] * (2 * nb), in_flight=nb)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
