@@ -417,8 +417,11 @@ __device__ __forceinline__ void jac_block(double2 r1, double2 r2, double x00, do
 // Runs the sweeps on the slot-ordered A in J (buffer 0 filled by the caller) and the
 // player-ordered V (m x m, row stride LV, initialised here). Returns the sweep count
 // (-1 without convergence) and in `fin` the buffer holding the final A.
-template <int JB, bool V_IN_LDS>
-__device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, int* flag, const double** fin) {
+// REPLAY: V is not formed here; the update threads of slot pair s (the first group) store the
+// round's (cos, sin) of pair s at rot[(sweep (m - 1) + round) np + s] for eigh_replay_kernel.
+template <int JB, bool V_IN_LDS, bool REPLAY = false>
+__device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, int* flag, const double** fin,
+                      double2* rot = nullptr) {
     const int m = c.m, np = m / 2, nb = np * (np + 1) / 2;
     double* A0 = J;
     double* A1 = J + 4 * nb;
@@ -451,9 +454,11 @@ __device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, in
         return hx ? (hy ? z11 : z10) : (hy ? z01 : z00);
     };
 
-    for (int e = c.tid; e < m * m; e += 512) {
-        const int i = e / m, j = e - (e / m) * m;
-        V[i * LV + j] = (i == j) ? 1.0 : 0.0;
+    if constexpr (!REPLAY) {
+        for (int e = c.tid; e < m * m; e += 512) {
+            const int i = e / m, j = e - (e / m) * m;
+            V[i * LV + j] = (i == j) ? 1.0 : 0.0;
+        }
     }
     auto big = [&](double v, double di, double dj) {
         return v != 0.0 && fabs(v) > JAC_EPS * sqrt(fabs(di) * fabs(dj));
@@ -544,10 +549,10 @@ __device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, in
                 // V in global memory: the eigenvector rows of this round are requested before
                 // the block work, so their latency overlaps it (the columns depend on the round
                 // only; the rotation is applied after the blocks)
-                double gvv[V_IN_LDS ? 1 : NVB * JV][2];
-                int gva[V_IN_LDS ? 1 : NVB * JV];
+                double gvv[V_IN_LDS || REPLAY ? 1 : NVB * JV][2];
+                int gva[V_IN_LDS || REPLAY ? 1 : NVB * JV];
                 int gvp = 0, gvq = 0;
-                if constexpr (!V_IN_LDS) {
+                if constexpr (!V_IN_LDS && !REPLAY) {
                     if (v_on) {
                         const int s_ = opaque(vS), g_ = opaque(vG);
                         gvp = player(s_, round, m); gvq = player(m - 1 - s_, round, m);
@@ -587,7 +592,9 @@ __device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, in
                         An[wa[u][0]] = z00; An[wa[u][1]] = z01; An[wa[u][2]] = z10; An[wa[u][3]] = z11;
                     }
                 }
-                if (!V_IN_LDS && v_on) {
+                if constexpr (REPLAY) {
+                    if (ut < np) rot[(sweep * (m - 1) + round) * np + ut] = cso[ut];
+                } else if (!V_IN_LDS && v_on) {
                     const double2 rv = cso[opaque(vS)];
 #pragma unroll
                     for (int k = 0; k < NVB * JV; ++k) {
@@ -835,12 +842,20 @@ struct EighArgs {
     int v_in_lds;
 };
 
+// Rotation record of the eigensolver (cos, sin per slot pair and round, MAX_SWEEPS sweeps of
+// mp - 1 rounds at most), then the eigenvalue ranks (mp ints), after lambda and the extras.
+__host__ __device__ __forceinline__ long long rot_record_doubles(int mp) {
+    return (long long)MAX_SWEEPS * (mp - 1) * mp + mp;
+}
 __host__ __device__ __forceinline__ long long target_doubles(int mp) {
+    return (long long)S_COUNT * mp * mp + mp + 16 + rot_record_doubles(mp);
+}
+__host__ __device__ __forceinline__ long long rot_record_offset(int mp) {
     return (long long)S_COUNT * mp * mp + mp + 16;
 }
 
 // Body of target b's eigensolver workgroup (ET threads, dynamic LDS eigh_lds_doubles).
-template <int JB, bool V_IN_LDS>
+template <int JB, bool V_IN_LDS, bool REPLAY = false>
 __device__ __forceinline__ void eigh_body(const EighArgs& a, int b)
 {
     extern __shared__ double lds[];
@@ -888,7 +903,9 @@ __device__ __forceinline__ void eigh_body(const EighArgs& a, int b)
     // V's dummy row (target of row indices past m): one extra LDS row, or slot T4 below T5
     double* Vj = V_IN_LDS ? J + jac_area(m) : ws + (long long)S_T5 * mp * mp;
     const int vdummy = V_IN_LDS ? m * LV : -(mp * mp);
-    const int sweeps = jacobi<JB, V_IN_LDS>(c, J, Vj, LV, vdummy, flag, &Af);
+    double2* rot = reinterpret_cast<double2*>(ws + rot_record_offset(mp));
+    int* granks = reinterpret_cast<int*>(ws + rot_record_offset(mp) + (long long)MAX_SWEEPS * (mp - 1) * mp);
+    const int sweeps = jacobi<JB, V_IN_LDS, REPLAY>(c, J, Vj, LV, vdummy, flag, &Af, rot);
     auto eig = [&](int x) {
         const int px = x < np ? x : m - 1 - x;
         const int bi = 2 * (px * np - (px * (px - 1)) / 2);
@@ -902,16 +919,19 @@ __device__ __forceinline__ void eigh_body(const EighArgs& a, int b)
             rank += (lj < li || (lj == li && j < i)) ? 1 : 0;
         }
         pairs[i] = rank;
+        if constexpr (REPLAY) granks[i] = rank;
     }
     sync();
     for (int i = c.tid; i < mp; i += ET) {
         if (i < m) lam[pairs[i]] = eig(i);
         else lam[i] = 1.0;
     }
-    for (int e = c.tid; e < mp * mp; e += ET) {
-        const int i = e / mp, j = e - (e / mp) * mp;
-        if (i < m && j < m) Vout[i * mp + pairs[j]] = Vj[i * LV + j];
-        else Vout[e] = 0.0;
+    if constexpr (!REPLAY) {
+        for (int e = c.tid; e < mp * mp; e += ET) {
+            const int i = e / mp, j = e - (e / mp) * mp;
+            if (i < m && j < m) Vout[i * mp + pairs[j]] = Vj[i * LV + j];
+            else Vout[e] = 0.0;
+        }
     }
     if (c.tid == 0) {
         extra[0] = scale;
@@ -921,11 +941,89 @@ __device__ __forceinline__ void eigh_body(const EighArgs& a, int b)
     }
 }
 
-template <int JB, bool V_IN_LDS>
+template <int JB, bool V_IN_LDS, bool REPLAY = false>
 __global__ __launch_bounds__(ET)
 void eigh_kernel(EighArgs a)
 {
-    eigh_body<JB, V_IN_LDS>(a, blockIdx.x);
+    eigh_body<JB, V_IN_LDS, REPLAY>(a, blockIdx.x);
+}
+
+// The eigenvectors from the eigensolver's rotation record. Row i of V only ever changes by the
+// column rotations, so every row is an independent replay of the same rounds: one wave per row,
+// 8 rows per workgroup, the round parameters staged through LDS for the 8 waves. Lane s holds the
+// row's entries at slots s and m - 1 - s (slot order: at round 0 of every sweep slot = column);
+// a round applies pair s's rotation with the eigensolver's expressions (the same operations on
+// the same values, so V is bitwise the in-kernel one) and then moves the tournament (the entry at
+// slot x goes to slot x + 1 over 1 .. m - 1, m - 1 to 1): top entries shift up one lane, bottom
+// entries down one lane. At the end the columns are ordered by the eigenvalue ranks.
+constexpr int RP_ROWS = 8;          // rows (waves) per workgroup
+constexpr int RP_CHUNK = 16;        // rounds staged per LDS chunk
+__global__ __launch_bounds__(RP_ROWS * WAVE)
+void eigh_replay_kernel(EighArgs a)
+{
+    __shared__ double2 par[2][RP_CHUNK * 64];
+    const int b = blockIdx.y;
+    const int m = a.m, mp = a.mp, np = m / 2;
+    const double* ws = a.ws + (long long)b * target_doubles(mp);
+    const double2* rot = reinterpret_cast<const double2*>(ws + rot_record_offset(mp));
+    const int* granks = reinterpret_cast<const int*>(ws + rot_record_offset(mp) + (long long)MAX_SWEEPS * (mp - 1) * mp);
+    const double* extra = ws + (long long)S_COUNT * mp * mp + mp;
+    double* Vout = const_cast<double*>(ws) + (long long)S_V * mp * mp;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * RP_ROWS + wave;                  // row of V (uniform per wave)
+    const int sw = (int)extra[1];
+    const int rounds = (sw < 0 ? MAX_SWEEPS : sw) * (m - 1);
+    const int s = lane < np ? lane : np - 1;                    // lanes past np mirror the last pair
+    double top = (i == s) ? 1.0 : 0.0, bot = (i == m - 1 - s) ? 1.0 : 0.0;
+    const int nchunks = (rounds + RP_CHUNK - 1) / RP_CHUNK;
+    auto stage = [&](int ch, int buf) {
+        for (int e = threadIdx.x; e < RP_CHUNK * np; e += RP_ROWS * WAVE) {
+            const int rr = e / np, ss = e - rr * np;
+            const int round = ch * RP_CHUNK + rr;
+            par[buf][rr * 64 + ss] = round < rounds ? rot[(long long)round * np + ss] : make_double2(1.0, 0.0);
+        }
+    };
+    // one round: pair rotation, then the tournament move (new top(s) = top(s - 1) for s >= 2,
+    // bot(0) for s = 1, top(0) for s = 0; new bot(s) = bot(s + 1) for s < np - 1, top(np - 1) for
+    // s = np - 1); lanes 0 and np - 1 are read with readlane, the neighbours with a permute
+    auto round_step = [&](double2 rv) {
+        const double t = fma(rv.x, top, -(rv.y * bot));
+        const double u = fma(rv.y, top, rv.x * bot);
+        if (np == 1) { top = t; bot = u; return; }      // m = 2: the two slots stay
+        const double up = __shfl(t, lane > 0 ? lane - 1 : 0);
+        const double dn = __shfl(u, lane + 1 < np ? lane + 1 : 0);
+        const double b0 = readlane_d(u, 0);
+        const double tl = readlane_d(t, np - 1);
+        top = s == 0 ? t : (s == 1 ? b0 : up);
+        bot = s == np - 1 ? tl : dn;
+    };
+    if (nchunks > 0) stage(0, 0);
+    for (int ch = 0; ch < nchunks; ++ch) {
+        __syncthreads();                                        // chunk ch staged; chunk ch - 1 read
+        if (ch + 1 < nchunks) stage(ch + 1, (ch + 1) & 1);
+        const double2* pc = par[ch & 1];
+        const int nr = min(RP_CHUNK, rounds - ch * RP_CHUNK);
+        if (nr == RP_CHUNK) {
+            double2 rv[RP_CHUNK];                               // the chunk's parameters in registers
+#pragma unroll
+            for (int rr = 0; rr < RP_CHUNK; ++rr) rv[rr] = pc[rr * 64 + s];
+#pragma unroll
+            for (int rr = 0; rr < RP_CHUNK; ++rr) round_step(rv[rr]);
+        } else {
+            for (int rr = 0; rr < nr; ++rr) round_step(pc[rr * 64 + s]);
+        }
+    }
+    if (i >= mp) return;
+    double* row = Vout + (long long)i * mp;
+    if (i < m) {
+        if (lane < np) {
+            row[granks[lane]] = top;
+            row[granks[m - 1 - lane]] = bot;
+        }
+        for (int j = m + lane; j < mp; j += 64) row[j] = 0.0;
+    } else {
+        for (int j = lane; j < mp; j += 64) row[j] = 0.0;
+    }
 }
 
 // ---------------------------------------------------------------- chained-HVP PCG
@@ -1917,8 +2015,32 @@ static size_t eigh_lds_doubles(int m, bool v_in_lds) {
     return 48 + (size_t)jac_area(m) + (v_in_lds ? (size_t)(m + 1) * (m + 1) : 0);   // V and its dummy row
 }
 
+#ifndef SSN_EIGH_REPLAY
+#define SSN_EIGH_REPLAY 1
+#endif
+
 static hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int ldc, int batch,
                               int m, double* ws, size_t lds_limit) {
+    if (SSN_EIGH_REPLAY) {
+        // rotations only in the per-target kernel, then the eigenvectors replayed row by row
+        const size_t lds = eigh_lds_doubles(m, false) * 8;
+        if (lds > lds_limit) return hipErrorInvalidValue;
+        EighArgs e;
+        e.Cr = Cr; e.sC_b = sC_b; e.ldc = ldc; e.m = m; e.mp = pad16(m); e.ws = ws; e.v_in_lds = 0;
+        const int np = m / 2;
+        const int blocks = (np * (np - 1) / 2 + JU - 1) / JU;
+        if (blocks > JB_MAX) return hipErrorInvalidValue;
+        const bool small = blocks <= JB_BATCH;
+        const void* fn = small ? (const void*)eigh_kernel<JB_BATCH, false, true> : (const void*)eigh_kernel<JB_MAX, false, true>;
+        hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (err != hipSuccess) return err;
+        if (small) hipLaunchKernelGGL((eigh_kernel<JB_BATCH, false, true>), dim3(batch), dim3(ET), lds, stream, e);
+        else hipLaunchKernelGGL((eigh_kernel<JB_MAX, false, true>), dim3(batch), dim3(ET), lds, stream, e);
+        err = hipGetLastError();
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL(eigh_replay_kernel, dim3(e.mp / RP_ROWS, batch), dim3(RP_ROWS * WAVE), 0, stream, e);
+        return hipGetLastError();
+    }
     const bool v_in_lds = eigh_lds_doubles(m, true) * 8 <= lds_limit;
     const size_t lds = eigh_lds_doubles(m, v_in_lds) * 8;
     if (lds > lds_limit) return hipErrorInvalidValue;
