@@ -344,7 +344,7 @@ def main():
                 "fp64_equivalent_tflops": flops / (ms * 1e-3) / 1e12, "avg_ms": ms}
 
     kernels = [roof_mfma("ssn::gram_partial_kernel<6> (+ gram_reduce_tiles_kernel)", f_gram, gram_ms),
-               roof_mfma("ssn::eigh_kernel<3,true>", f_eigh, eigh_ms),
+               roof_mfma("ssn::eigh_kernel<3,false,true> (+ eigh_replay_kernel)", f_eigh, eigh_ms),
                roof_mfma("ssn::dual_kernel<6>", fd, newton_ms),
                roof_mfma("ssn::recover_lds_kernel<6>", f_rec, rec_ms)]
     kernels_split = [roof_hbm("ssn::split_gram_kernel<6,2,5> (+ gram_reduce_tiles_kernel)", b_gram, f_gram,
@@ -353,7 +353,7 @@ def main():
                               stages_split["recover"])]
     # HBM traffic per launch of each kernel (the main kernel of a stage) at this workload's grid
     for k, (pname, grid) in zip(kernels, (("void ssn::gram_partial_kernel<6>", 4096 * BATCH),
-                                          ("void ssn::eigh_kernel<3, true>", 512 * BATCH),
+                                          ("void ssn::eigh_kernel<3, false, true>", 512 * BATCH),
                                           ("void ssn::dual_kernel<6>", 512 * BATCH),
                                           ("void ssn::recover_lds_kernel<6>", 2048 * BATCH))):
         k["traffic"] = profiled_traffic(pname, grid)
