@@ -958,19 +958,19 @@ void eigh_kernel(EighArgs a)
 // entries down one lane. At the end the columns are ordered by the eigenvalue ranks.
 constexpr int RP_ROWS = 8;          // rows (waves) per workgroup
 constexpr int RP_CHUNK = 16;        // rounds staged per LDS chunk
-__global__ __launch_bounds__(RP_ROWS * WAVE)
-void eigh_replay_kernel(EighArgs a)
+constexpr int MAX_ORDER_PADDED = 128; // largest padded order (NT = 8)
+// Rows [8 blk, 8 blk + 8) of the target whose solver block is ws (order m, padded mp); the whole
+// workgroup returns at once when blk is past the target's rows.
+__device__ __forceinline__ void eigh_replay_body(const double* ws, int m, int mp, int blk, double2 (*par)[RP_CHUNK * 64])
 {
-    __shared__ double2 par[2][RP_CHUNK * 64];
-    const int b = blockIdx.y;
-    const int m = a.m, mp = a.mp, np = m / 2;
-    const double* ws = a.ws + (long long)b * target_doubles(mp);
+    if (blk * RP_ROWS >= mp) return;
+    const int np = m / 2;
     const double2* rot = reinterpret_cast<const double2*>(ws + rot_record_offset(mp));
     const int* granks = reinterpret_cast<const int*>(ws + rot_record_offset(mp) + (long long)MAX_SWEEPS * (mp - 1) * mp);
     const double* extra = ws + (long long)S_COUNT * mp * mp + mp;
     double* Vout = const_cast<double*>(ws) + (long long)S_V * mp * mp;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int i = blockIdx.x * RP_ROWS + wave;                  // row of V (uniform per wave)
+    const int i = blk * RP_ROWS + wave;                         // row of V (uniform per wave)
     const int sw = (int)extra[1];
     const int rounds = (sw < 0 ? MAX_SWEEPS : sw) * (m - 1);
     const int s = lane < np ? lane : np - 1;                    // lanes past np mirror the last pair
@@ -1024,6 +1024,13 @@ void eigh_replay_kernel(EighArgs a)
     } else {
         for (int j = lane; j < mp; j += 64) row[j] = 0.0;
     }
+}
+
+__global__ __launch_bounds__(RP_ROWS * WAVE)
+void eigh_replay_kernel(EighArgs a)
+{
+    __shared__ double2 par[2][RP_CHUNK * 64];
+    eigh_replay_body(a.ws + (long long)blockIdx.y * target_doubles(a.mp), a.m, a.mp, blockIdx.x, par);
 }
 
 // ---------------------------------------------------------------- chained-HVP PCG
@@ -2115,19 +2122,27 @@ hipError_t launch_dual(hipStream_t stream, const double* Cr, const double* Ci, l
 }
 
 // ---------------------------------------------------------------- uneven batches
-// One workgroup per target of the plan; each runs the body of its own order / eigensolver
-// variant. The dynamic LDS of a launch is the largest of its targets' needs.
+// One workgroup per target of the plan; each runs the rotation-only body of its own order
+// (the eigenvectors follow in eigh_replay_uneven_kernel). The dynamic LDS of a launch is the
+// largest of its targets' needs, which covers the rotation-only body.
 __global__ __launch_bounds__(ET)
 void eigh_uneven_kernel(const UTarget* __restrict__ tg)
 {
     const UTarget& t = tg[blockIdx.x];
     EighArgs a;
-    a.Cr = t.Cr; a.sC_b = 0; a.ldc = t.m; a.m = t.m; a.mp = t.mp; a.ws = t.dual; a.v_in_lds = t.eigh_vlds;
-    if (t.eigh_small) {
-        if (t.eigh_vlds) eigh_body<JB_BATCH, true>(a, 0); else eigh_body<JB_BATCH, false>(a, 0);
-    } else {
-        if (t.eigh_vlds) eigh_body<JB_MAX, true>(a, 0); else eigh_body<JB_MAX, false>(a, 0);
-    }
+    a.Cr = t.Cr; a.sC_b = 0; a.ldc = t.m; a.m = t.m; a.mp = t.mp; a.ws = t.dual; a.v_in_lds = 0;
+    if (t.eigh_small) eigh_body<JB_BATCH, false, true>(a, 0);
+    else eigh_body<JB_MAX, false, true>(a, 0);
+}
+
+// Eigenvector replay of every target of the plan: blockIdx.y = target, blockIdx.x = row block
+// (grid sized for the largest order; blocks past a target's rows return at once).
+__global__ __launch_bounds__(RP_ROWS * WAVE)
+void eigh_replay_uneven_kernel(const UTarget* __restrict__ tg)
+{
+    __shared__ double2 par[2][RP_CHUNK * 64];
+    const UTarget& t = tg[blockIdx.y];
+    eigh_replay_body(t.dual, t.m, t.mp, blockIdx.x, par);
 }
 
 __global__ __launch_bounds__(DT)
@@ -2181,6 +2196,10 @@ hipError_t launch_dual_uneven(hipStream_t stream, const UTarget* tg, int count, 
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)eigh_lds);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(eigh_uneven_kernel, dim3(count), dim3(ET), eigh_lds, stream, tg);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(eigh_replay_uneven_kernel, dim3(MAX_ORDER_PADDED / RP_ROWS, count), dim3(RP_ROWS * WAVE), 0,
+                       stream, tg);
     err = hipGetLastError();
     if (err != hipSuccess) return err;
     err = hipFuncSetAttribute((const void*)dual_uneven_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
