@@ -985,13 +985,14 @@ __device__ __forceinline__ void eigh_replay_body(const double* ws, int m, int mp
     };
     // one round: pair rotation, then the tournament move (new top(s) = top(s - 1) for s >= 2,
     // bot(0) for s = 1, top(0) for s = 0; new bot(s) = bot(s + 1) for s < np - 1, top(np - 1) for
-    // s = np - 1); lanes 0 and np - 1 are read with readlane, the neighbours with a permute
+    // s = np - 1); lanes 0 and np - 1 are read with readlane, the neighbours with whole-wave DPP
+    // shifts (wave_shr:1 / wave_shl:1; the lanes without a source do not use the value)
     auto round_step = [&](double2 rv) {
         const double t = fma(rv.x, top, -(rv.y * bot));
         const double u = fma(rv.y, top, rv.x * bot);
         if (np == 1) { top = t; bot = u; return; }      // m = 2: the two slots stay
-        const double up = __shfl(t, lane > 0 ? lane - 1 : 0);
-        const double dn = __shfl(u, lane + 1 < np ? lane + 1 : 0);
+        const double up = dpp<0x138, 0xF, 0xF>(t);      // t of lane - 1
+        const double dn = dpp<0x130, 0xF, 0xF>(u);      // u of lane + 1
         const double b0 = readlane_d(u, 0);
         const double tl = readlane_d(t, np - 1);
         top = s == 0 ? t : (s == 1 ? b0 : up);
