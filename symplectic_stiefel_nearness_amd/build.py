@@ -8,16 +8,18 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libssn.so"
-SOURCES = ["gram.hip", "recover.hip", "split.hip", "dual.hip", "capi.hip"]
+SOURCES = ["dual_uneven.hip", "dual.hip", "adjoint.hip", "eigh.hip", "recover.hip", "gram.hip", "split.hip",
+           "capi.hip"]   # longest compile first
 ARCH = os.environ.get("SSN_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # keep hipcc from emitting packed-FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
@@ -31,15 +33,41 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found; a ROCm toolchain is required to build libssn.so")
 
 
+def _jobs() -> int:
+    """Parallel compiles: the process's CPU share, at most 8 (one per source)."""
+    try:
+        cpus = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = os.cpu_count() or 1
+    return max(1, min(len(SOURCES), cpus))
+
+
 def build(verbose: bool = False) -> Path:
-    """Compile every HIP source into libssn.so (always recompiles)."""
+    """Compile every HIP source into libssn.so (always recompiles; one object per source,
+    compiled in parallel, then linked)."""
+    objdir = HERE / "_build"
+    objdir.mkdir(exist_ok=True)
+    cc = hipcc()
+
+    def compile_one(src: str) -> Path:
+        obj = objdir / (Path(src).stem + ".o")
+        cmd = [cc, *HIPCC_FLAGS, "-c", "-I", str(CSRC), str(CSRC / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src} ({res.returncode}):\n{res.stderr[-6000:]}")
+        return obj
+
+    with ThreadPoolExecutor(max_workers=_jobs()) as pool:
+        objs = list(pool.map(compile_one, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(CSRC), *[str(CSRC / s) for s in SOURCES], "-o", str(tmp)]
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *[str(o) for o in objs], "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stderr[-6000:]}")
+        raise RuntimeError(f"hipcc link failed ({res.returncode}):\n{res.stderr[-6000:]}")
     os.replace(tmp, LIB)
     return LIB
 
