@@ -33,7 +33,7 @@ extern "C" {
 /* per-target status values written by ssn_dual / ssn_solve */
 #define SSN_ST_OK 0
 #define SSN_ST_NO_DESCENT 1      /* Armijo backtracking exhausted (dual_recommended_hvp.py:189-190) */
-#define SSN_ST_NEWTON_CAP 2      /* MAX_NEWTON = 100 reached (dual_recommended_hvp.py:191)          */
+#define SSN_ST_NEWTON_CAP 2      /* Newton cap (MAX_NEWTON = 100) reached (dual_recommended_hvp.py:191) */
 #define SSN_ST_NONFINITE 3       /* non-finite residual                                             */
 #define SSN_ST_EIGH 4            /* Jacobi eigensolver did not converge                             */
 
@@ -54,7 +54,8 @@ const char* ssn_status_string(int status);
 int ssn_host_copy(void* dst, const void* src, size_t bytes, int threads);
 
 /* Bytes of device scratch needed by ssn_solve (and sufficient for ssn_gram, ssn_dual,
- * ssn_recover) for `batch` targets of shape 2n x m. 0 for unsupported shapes. */
+ * ssn_recover) for `batch` targets of shape 2n x m. 0 for unsupported shapes. Non-decreasing in
+ * `batch`: a workspace sized for a batch serves every smaller batch of the same shape. */
 size_t ssn_workspace_bytes(int batch, int n, int m);
 
 /* Cr = A^T A + D^T D, Ci = A^T D - D^T A for T = [A; D] ([batch][2n][m], strides sT_b, sT_r).
@@ -104,6 +105,30 @@ int ssn_recover(void* stream, const double* T, long long sT_b, long long sT_r, c
 int ssn_solve(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
               double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
               long long* iters, int* status, int* counters, void* ws, size_t ws_bytes);
+
+/* Iteration caps and stopping tolerance of the dual iteration, and the precision of the Gram and
+ * recovery, for ssn_solve_opts. The reference fixes the caps and the tolerance as module
+ * constants (dual_recommended_hvp.py:15-16: FEAS_TOL = 1e-8, MAX_NEWTON = 100, MAX_CG = 150,
+ * MAX_BACKTRACK = 40); ssn_default_options() fills those values, with which ssn_solve_opts is
+ * ssn_solve (split = 0) or ssn_solve_split (split = 1). */
+typedef struct ssn_options {
+    int max_newton;      /* Newton updates, 0 .. 2^20; reaching it sets SSN_ST_NEWTON_CAP (0: at once)      */
+    int max_cg;          /* CG iterations per Newton step, 0 .. 2^20 (0: zero direction, so the step
+                            falls back to the preconditioned gradient, dual_recommended_hvp.py:163-164) */
+    int max_backtrack;   /* Armijo trials per Newton step, 0 .. 1100; exhausting them sets SSN_ST_NO_DESCENT */
+    int split;           /* 0: FP64 Gram and recovery; 1: split precision (ssn_solve_split)                */
+    double feas_tol;     /* a target stops once its scaled residual < feas_tol / 2; finite, >= 0           */
+} ssn_options;
+
+void ssn_default_options(ssn_options* opts);
+
+/* ssn_solve with the caps, tolerance and precision of `opts` (host pointer, read during the call).
+ * SSN_ERR_ARGS for a null or out-of-range `opts`. Replaces solve (dual_recommended_hvp.py:231-242)
+ * run with other values of the module constants of :15-16. */
+int ssn_solve_opts(void* stream, const ssn_options* opts, const double* T, long long sT_b, long long sT_r,
+                   int batch, int n, int m, double* X, long long sX_b, long long sX_r, double* omega,
+                   long long sO_b, int ldo, long long* iters, int* status, int* counters, void* ws,
+                   size_t ws_bytes);
 
 /* Split-precision forms of ssn_gram, ssn_recover and ssn_solve (same arguments and workspace).
  * No reference counterpart (the reference's Gram and recovery are FP64 only,

@@ -195,14 +195,20 @@ class DualStats:
     failed: np.ndarray = field(default=None)
 
 
-def dual_batch(Cr, Ci, raise_on_failure=True, feas_tol=None):
+def dual_batch(Cr, Ci, raise_on_failure=True, feas_tol=None, max_newton=None, max_cg=None,
+               max_backtrack=None):
     """Newton / PCG / Armijo on the dual (dual_recommended_hvp.py:99-191).
 
     Returns (omega, L, iterations, P, Q, stats); P, Q are the final inverse parts in
-    the original basis (the reference's `final` cache, :131-133). `feas_tol` overrides
-    FEAS_TOL (the tests tighten it to converge the dual to rounding level).
+    the original basis (the reference's `final` cache, :131-133). `feas_tol`, `max_newton`,
+    `max_cg` and `max_backtrack` override the module constants FEAS_TOL, MAX_NEWTON, MAX_CG and
+    MAX_BACKTRACK (:15-16) for this call (the tests tighten the tolerance to converge the dual
+    to rounding level, and lower the caps to reach the failure branches).
     """
     feas_tol = FEAS_TOL if feas_tol is None else feas_tol
+    max_newton = MAX_NEWTON if max_newton is None else max_newton
+    max_cg = MAX_CG if max_cg is None else max_cg
+    max_backtrack = MAX_BACKTRACK if max_backtrack is None else max_backtrack
     batch, m, _ = Cr.shape
     I = np.broadcast_to(np.eye(m), (batch, m, m))
     omega = np.zeros_like(Cr)
@@ -222,7 +228,7 @@ def dual_batch(Cr, Ci, raise_on_failure=True, feas_tol=None):
                       np.zeros(batch, np.int64), np.zeros(batch, np.int64),
                       np.zeros(batch, bool))
     Q = np.zeros_like(omega)
-    for newton_step in range(MAX_NEWTON):                                       # :118
+    for newton_step in range(max_newton):                                       # :118
         if newton_step == 0:
             Q = np.zeros_like(omega)
             Sr = np.zeros_like(Cr)
@@ -253,7 +259,7 @@ def dual_batch(Cr, Ci, raise_on_failure=True, feas_tol=None):
             p, rz = z.copy(), inner(r, z)
             tolerance = np.clip(np.sqrt(residual), 1e-8, 0.1) * gradient_norm
             cg_active = active.copy()
-            for _ in range(MAX_CG):
+            for _ in range(max_cg):
                 Hp = hvp(P, Q, Sr, Si, p)
                 stats.hvps += cg_active
                 stats.cg += cg_active
@@ -275,7 +281,7 @@ def dual_batch(Cr, Ci, raise_on_failure=True, feas_tol=None):
         slope = inner(gradient, direction)                                      # :167
         step = np.ones_like(slope)
         pending = active.copy()
-        for _ in range(MAX_BACKTRACK):                                          # :170
+        for _ in range(max_backtrack):                                          # :170
             stats.backtracks += pending
             delta = np.where(pending, step, 0.0)[:, None, None] * direction
             trial = skew(omega + delta)
@@ -302,7 +308,10 @@ def dual_batch(Cr, Ci, raise_on_failure=True, feas_tol=None):
     stats.failed |= active
     if raise_on_failure:
         raise RuntimeError(MSG_NEWTON_CAP)
-    return None
+    # cap reached without raising: the state after the last update, as the GPU reports it
+    omega, L, P, Q = from_eigenbasis(omega, P, Q, vectors)
+    stats.newton[:] = iterations
+    return omega, L, iterations, P, Q, stats
 
 
 def recover(T, P, Q):
@@ -312,13 +321,18 @@ def recover(T, P, Q):
     return np.concatenate((top, bot), axis=-2)
 
 
-def solve(T, return_stats=False, feas_tol=None):
-    """(X, omega, iterations) for targets T [batch, 2n, 2k] (dual_recommended_hvp.py:231-242)."""
+def solve(T, return_stats=False, feas_tol=None, raise_on_failure=True, **caps):
+    """(X, omega, iterations) for targets T [batch, 2n, 2k] (dual_recommended_hvp.py:231-242).
+
+    caps: max_newton, max_cg, max_backtrack, as in dual_batch. raise_on_failure=False returns
+    the state reached when the Newton cap is hit instead of raising (the line-search failure
+    still drops the failed targets from the active set, as dual_batch does)."""
     T = np.asarray(T, dtype=np.float64)
     if T.ndim != 3 or T.shape[-2] % 2:
         raise ValueError("T must be [batch, 2n, 2k]")
     Cr, Ci = gram(T)
-    omega, L, iterations, P, Q, stats = dual_batch(Cr, Ci, feas_tol=feas_tol)
+    omega, L, iterations, P, Q, stats = dual_batch(Cr, Ci, raise_on_failure=raise_on_failure,
+                                                   feas_tol=feas_tol, **caps)
     X = recover(T, P, Q)
     if return_stats:
         return X, omega, iterations, stats

@@ -37,6 +37,16 @@ class SsnTarget(ctypes.Structure):
 
 _TP = ctypes.POINTER(SsnTarget)
 
+
+class SsnOptions(ctypes.Structure):
+    """Mirror of `ssn_options` (include/ssn.h)."""
+    _fields_ = [("max_newton", ctypes.c_int), ("max_cg", ctypes.c_int),
+                ("max_backtrack", ctypes.c_int), ("split", ctypes.c_int),
+                ("feas_tol", ctypes.c_double)]
+
+
+_OP = ctypes.POINTER(SsnOptions)
+
 # name -> (restype, argtypes); must match include/ssn.h exactly
 SIGNATURES = {
     "ssn_init": (_I, []),
@@ -52,6 +62,8 @@ SIGNATURES = {
     "ssn_dual_newton": (_I, [_P, _P, _P, _L, _I, _I, _I, _P, _L, _I, _P, _P, _P, _P, _P, _P, _S]),
     "ssn_recover": (_I, [_P, _P, _L, _L, _P, _P, _I, _I, _I, _P, _L, _L, _P, _S]),
     "ssn_solve": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _L, _I, _P, _P, _P, _P, _S]),
+    "ssn_default_options": (None, [_OP]),
+    "ssn_solve_opts": (_I, [_P, _OP, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _L, _I, _P, _P, _P, _P, _S]),
     "ssn_backward": (_I, [_P, _P, _L, _L, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _P, _S]),
     "ssn_gram_split": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _P, _L, _I, _P, _S]),
     "ssn_recover_split": (_I, [_P, _P, _L, _L, _P, _P, _I, _I, _I, _P, _L, _L, _P, _S]),

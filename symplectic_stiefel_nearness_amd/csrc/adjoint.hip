@@ -107,7 +107,8 @@ void adjoint_kernel(AdjArgs a)
     if constexpr (ChainCfg<NT>::OK) {
         int how;
         const int nh = cg_chained<NT, true>(c, lds, [&](int i, int j) { return ((T1[i * MP + j] - T1[j * MP + i]) / 2) / scale; },
-                                            Sr, Si, Pe, slot(S_QT), lam_s, a.tol, X, nullptr, how);
+                                            Sr, Si, Pe, slot(S_QT), lam_s, a.tol, X, nullptr, how,
+                                            MAX_CG);
         it = how == 0 ? nh - 1 : nh;
         status = how == 1 ? 0 : 1;
     }
@@ -129,7 +130,7 @@ void adjoint_kernel(AdjArgs a)
 template <int NT>
 static hipError_t launch_adj_nt(hipStream_t stream, AdjArgs a, int batch, size_t lds_limit) {
     const size_t lds = (size_t)Cfg<NT>::LDS_END * 8;
-    if (lds > lds_limit) return hipErrorInvalidValue;
+    if (lds > lds_limit) return LAUNCH_UNSUPPORTED;
     hipError_t err = hipFuncSetAttribute((const void*)adjoint_kernel<NT>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
@@ -152,7 +153,7 @@ hipError_t launch_adjoint(hipStream_t stream, double* ws, const double* Pf, cons
         case 6: return launch_adj_nt<6>(stream, a, batch, lds_limit);
         case 7: return launch_adj_nt<7>(stream, a, batch, lds_limit);
         case 8: return launch_adj_nt<8>(stream, a, batch, lds_limit);
-        default: return hipErrorInvalidValue;
+        default: return LAUNCH_UNSUPPORTED;
     }
 }
 

@@ -23,7 +23,8 @@ hipError_t launch_recover(hipStream_t, const double*, long long, long long, cons
                           double*, long long, long long);
 size_t dual_workspace_doubles(int batch, int m);
 hipError_t launch_dual(hipStream_t, const double*, const double*, long long, int, int, int,
-                       double*, long long, int, double*, double*, long long*, int*, int*, double*, size_t);
+                       double*, long long, int, double*, double*, long long*, int*, int*, double*, size_t,
+                       const Caps&);
 hipError_t launch_dual_eigh(hipStream_t, const double*, long long, int, int, int, double*, size_t);
 hipError_t launch_cross_gram(hipStream_t, const double*, long long, long long, const double*, long long, long long,
                              int, int, int, int, double*, double*, double*, long long);
@@ -33,7 +34,8 @@ hipError_t launch_backward_stream(hipStream_t, const double*, long long, long lo
                                   const double*, double*, int, int, int, double*, long long, long long);
 long long workspace_target_doubles(int m);
 hipError_t launch_dual_newton(hipStream_t, const double*, const double*, long long, int, int, int,
-                              double*, long long, int, double*, double*, long long*, int*, int*, double*, size_t);
+                              double*, long long, int, double*, double*, long long*, int*, int*, double*, size_t,
+                              const Caps&);
 hipError_t launch_split_gram(hipStream_t, const double*, long long, long long, int, int, int, int, double*);
 long long split_fragment_doubles(int m);
 hipError_t launch_split_recover(hipStream_t, const double*, long long, long long, const double*, const double*,
@@ -83,8 +85,9 @@ int current_device_ready(size_t* lds) {
 
 int hip_status(hipError_t e) {
     if (e == hipSuccess) return SSN_OK;
+    if (e == LAUNCH_UNSUPPORTED) return SSN_ERR_UNSUPPORTED;
     g_last_hip = e;
-    return e == hipErrorInvalidValue ? SSN_ERR_UNSUPPORTED : SSN_ERR_HIP;
+    return SSN_ERR_HIP;
 }
 
 bool shape_ok(int batch, int n, int m) {
@@ -98,12 +101,25 @@ struct Layout {
     int chunks;
 };
 
+// Gram partial slabs (batch x chunks) needed by any batch of at most `batch` targets: the
+// chunk count of a batch falls as the batch grows, so batch * chunks alone is not monotonic
+// (51 targets take 561 slabs, 52 take 520). Sizing for the worst smaller batch makes
+// ssn_workspace_bytes() non-decreasing in the batch, so a workspace sized for a batch also
+// serves every smaller one. A batch of at least the Gram's target workgroup count (512, at
+// most 2048 when tuned) takes one chunk per target, so batches past 2048 need no search.
+size_t partial_slabs(int batch, int n) {
+    size_t worst = (size_t)batch * gram_chunk_count(batch > 0 ? batch : 1, n);
+    const int lim = std::min(batch, 2048);
+    for (int b = 1; b <= lim; ++b) worst = std::max(worst, (size_t)b * gram_chunk_count(b, n));
+    return worst;
+}
+
 Layout layout(int batch, int n, int m) {
     Layout L;
     const size_t mp = (size_t)pad16(m);
     L.chunks = gram_chunk_count(batch > 0 ? batch : 1, n);
     size_t off = 0;
-    L.partial = off; off += (size_t)batch * L.chunks * 2 * mp * mp;
+    L.partial = off; off += partial_slabs(batch, n) * 2 * mp * mp;
     L.cr = off;      off += (size_t)batch * m * m;
     L.ci = off;      off += (size_t)batch * m * m;
     L.dual = off;    off += dual_workspace_doubles(batch, m);
@@ -201,7 +217,7 @@ int ssn_dual(void* stream, const double* Cr, const double* Ci, long long sC_b, i
     if (rc) return rc;
     if (!ws || ws_bytes < dual_workspace_doubles(batch, m) * sizeof(double)) return SSN_ERR_WORKSPACE;
     return hip_status(launch_dual((hipStream_t)stream, Cr, Ci, sC_b, ldc, batch, m, omega, sO_b, ldo,
-                                  P, Q, iters, status, counters, (double*)ws, lds));
+                                  P, Q, iters, status, counters, (double*)ws, lds, Caps()));
 }
 
 int ssn_dual_eigh(void* stream, const double* Cr, long long sC_b, int ldc, int batch, int m,
@@ -228,7 +244,7 @@ int ssn_dual_newton(void* stream, const double* Cr, const double* Ci, long long 
     if (rc) return rc;
     if (!ws || ws_bytes < dual_workspace_doubles(batch, m) * sizeof(double)) return SSN_ERR_WORKSPACE;
     return hip_status(launch_dual_newton((hipStream_t)stream, Cr, Ci, sC_b, ldc, batch, m, omega, sO_b, ldo,
-                                         P, Q, iters, status, counters, (double*)ws, lds));
+                                         P, Q, iters, status, counters, (double*)ws, lds, Caps()));
 }
 
 int ssn_recover(void* stream, const double* T, long long sT_b, long long sT_r, const double* P,
@@ -244,7 +260,7 @@ int ssn_recover(void* stream, const double* T, long long sT_b, long long sT_r, c
     if (!ws || ws_bytes < need) return SSN_ERR_WORKSPACE;
     hipStream_t s = (hipStream_t)stream;
     const hipError_t e = launch_recover_pq(s, T, sT_b, sT_r, P, Q, batch, n, m, X, sX_b, sX_r, lds);
-    if (e != hipErrorNotSupported) return hip_status(e);
+    if (e != LAUNCH_DECLINED) return hip_status(e);
     rc = hip_status(launch_recovery_fragments(s, P, Q, batch, m, (double*)ws));
     if (rc) return rc;
     return hip_status(launch_recover(s, T, sT_b, sT_r, (const double*)ws, batch, n, m, X, sX_b, sX_r));
@@ -264,9 +280,10 @@ int ssn_recover_split(void* stream, const double* T, long long sT_b, long long s
                                            X, sX_b, sX_r));
 }
 
-static int solve_impl(bool split, void* stream, const double* T, long long sT_b, long long sT_r, int batch,
-                      int n, int m, double* X, long long sX_b, long long sX_r, double* omega, long long sO_b,
-                      int ldo, long long* iters, int* status, int* counters, void* ws, size_t ws_bytes)
+static int solve_impl(bool split, const Caps& caps, void* stream, const double* T, long long sT_b, long long sT_r,
+                      int batch, int n, int m, double* X, long long sX_b, long long sX_r, double* omega,
+                      long long sO_b, int ldo, long long* iters, int* status, int* counters, void* ws,
+                      size_t ws_bytes)
 {
     if (!shape_ok(batch, n, m) || !T || !X || !omega || !iters || !status || ldo < m) return SSN_ERR_ARGS;
     if (batch == 0) return SSN_OK;
@@ -284,13 +301,13 @@ static int solve_impl(bool split, void* stream, const double* T, long long sT_b,
                                        (long long)m * m, m));
     if (rc) return rc;
     rc = hip_status(launch_dual(s, w + L.cr, w + L.ci, (long long)m * m, m, batch, m, omega, sO_b, ldo,
-                                w + L.pf, w + L.qf, iters, status, counters, w + L.dual, lds));
+                                w + L.pf, w + L.qf, iters, status, counters, w + L.dual, lds, caps));
     if (rc) return rc;
     if (split)
         return hip_status(launch_split_recover(s, T, sT_b, sT_r, w + L.pf, w + L.qf, w + L.frag, batch, n, m,
                                                X, sX_b, sX_r));
     const hipError_t e = launch_recover_pq(s, T, sT_b, sT_r, w + L.pf, w + L.qf, batch, n, m, X, sX_b, sX_r, lds);
-    if (e != hipErrorNotSupported) return hip_status(e);
+    if (e != LAUNCH_DECLINED) return hip_status(e);
     rc = hip_status(launch_recovery_fragments(s, w + L.pf, w + L.qf, batch, m, w + L.frag));
     if (rc) return rc;
     return hip_status(launch_recover(s, T, sT_b, sT_r, w + L.frag, batch, n, m, X, sX_b, sX_r));
@@ -300,16 +317,47 @@ int ssn_solve(void* stream, const double* T, long long sT_b, long long sT_r, int
               double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
               long long* iters, int* status, int* counters, void* ws, size_t ws_bytes)
 {
-    return solve_impl(false, stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo, iters, status,
-                      counters, ws, ws_bytes);
+    return solve_impl(false, Caps(), stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo, iters,
+                      status, counters, ws, ws_bytes);
 }
 
 int ssn_solve_split(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
                     double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
                     long long* iters, int* status, int* counters, void* ws, size_t ws_bytes)
 {
-    return solve_impl(true, stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo, iters, status,
-                      counters, ws, ws_bytes);
+    return solve_impl(true, Caps(), stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo, iters,
+                      status, counters, ws, ws_bytes);
+}
+
+void ssn_default_options(ssn_options* opts)
+{
+    if (!opts) return;
+    const Caps d;
+    opts->max_newton = d.max_newton;
+    opts->max_cg = d.max_cg;
+    opts->max_backtrack = d.max_backtrack;
+    opts->split = 0;
+    opts->feas_tol = d.feas_tol;
+}
+
+int ssn_solve_opts(void* stream, const ssn_options* opts, const double* T, long long sT_b, long long sT_r,
+                   int batch, int n, int m, double* X, long long sX_b, long long sX_r, double* omega,
+                   long long sO_b, int ldo, long long* iters, int* status, int* counters, void* ws,
+                   size_t ws_bytes)
+{
+    if (!opts) return SSN_ERR_ARGS;
+    constexpr int CAP_LIMIT = 1 << 20;
+    if (opts->max_newton < 0 || opts->max_newton > CAP_LIMIT || opts->max_cg < 0 || opts->max_cg > CAP_LIMIT ||
+        opts->max_backtrack < 0 || opts->max_backtrack > 1100 || (opts->split != 0 && opts->split != 1) ||
+        !(opts->feas_tol >= 0.0) || !(opts->feas_tol < 1e300))
+        return SSN_ERR_ARGS;
+    Caps caps;
+    caps.max_newton = opts->max_newton;
+    caps.max_cg = opts->max_cg;
+    caps.max_backtrack = opts->max_backtrack;
+    caps.feas_tol = opts->feas_tol;
+    return solve_impl(opts->split != 0, caps, stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo,
+                      iters, status, counters, ws, ws_bytes);
 }
 
 int ssn_backward(void* stream, const double* T, long long sT_b, long long sT_r,

@@ -60,6 +60,22 @@ __device__ __forceinline__ double wave_sum(double v) {
     return readlane_d(v, 63);
 }
 
+// Launcher results that are not HIP errors. Both values lie outside HIP's error enumeration,
+// so no HIP call returns them: LAUNCH_UNSUPPORTED says the shape does not fit this device
+// (the C ABI reports SSN_ERR_UNSUPPORTED); LAUNCH_DECLINED says a launcher does not handle
+// the shape and the caller takes another path. Every other value is a HIP error (SSN_ERR_HIP).
+constexpr hipError_t LAUNCH_UNSUPPORTED = static_cast<hipError_t>(2000);
+constexpr hipError_t LAUNCH_DECLINED = static_cast<hipError_t>(2001);
+
+// Iteration caps and the stopping tolerance of the Newton / PCG / Armijo iteration; the
+// defaults are the reference's constants (dual_recommended_hvp.py:15-16).
+struct Caps {
+    int max_newton = 100;       // Newton updates; reaching the cap is ST_NEWTON_CAP
+    int max_cg = 150;           // CG iterations per Newton step
+    int max_backtrack = 40;     // Armijo trials per Newton step; exhausting them is ST_NO_DESCENT
+    double feas_tol = 1e-8;     // a target stops when its scaled residual < feas_tol / 2
+};
+
 // Status codes shared by kernels and the C ABI (include/ssn.h mirrors them).
 enum Status : int {
     ST_OK = 0,

@@ -24,7 +24,7 @@ size_t dual_workspace_doubles(int batch, int m) {
 template <int NT>
 static hipError_t launch_nt(hipStream_t stream, DualArgs a, int batch, size_t lds_limit) {
     const size_t lds = (size_t)Cfg<NT>::LDS_END * 8;
-    if (lds > lds_limit) return hipErrorInvalidValue;
+    if (lds > lds_limit) return LAUNCH_UNSUPPORTED;
     hipError_t err = hipFuncSetAttribute((const void*)dual_kernel<NT>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (err != hipSuccess) return err;
@@ -35,10 +35,11 @@ static hipError_t launch_nt(hipStream_t stream, DualArgs a, int batch, size_t ld
 hipError_t launch_dual_newton(hipStream_t stream, const double* Cr, const double* Ci, long long sC_b, int ldc,
                               int batch, int m, double* omega, long long sO_b, int ldo,
                               double* Pf, double* Qf, long long* iters, int* status, int* counters,
-                              double* ws, size_t lds_limit)
+                              double* ws, size_t lds_limit, const Caps& caps)
 {
     const int mp = pad16(m);
     DualArgs a;
+    a.caps = caps;
     a.Cr = Cr; a.Ci = Ci; a.sC_b = sC_b; a.ldc = ldc; a.m = m; a.mp = mp;
     a.omega = omega; a.sO_b = sO_b; a.ldo = ldo; a.Pf = Pf; a.Qf = Qf;
     a.iters = iters; a.status = status; a.counters = counters; a.ws = ws; a.v_in_lds = 0;
@@ -51,19 +52,19 @@ hipError_t launch_dual_newton(hipStream_t stream, const double* Cr, const double
         case 6: return launch_nt<6>(stream, a, batch, lds_limit);
         case 7: return launch_nt<7>(stream, a, batch, lds_limit);
         case 8: return launch_nt<8>(stream, a, batch, lds_limit);
-        default: return hipErrorInvalidValue;
+        default: return LAUNCH_UNSUPPORTED;
     }
 }
 
 hipError_t launch_dual(hipStream_t stream, const double* Cr, const double* Ci, long long sC_b, int ldc,
                        int batch, int m, double* omega, long long sO_b, int ldo,
                        double* Pf, double* Qf, long long* iters, int* status, int* counters,
-                       double* ws, size_t lds_limit)
+                       double* ws, size_t lds_limit, const Caps& caps)
 {
     hipError_t err = launch_eigh(stream, Cr, sC_b, ldc, batch, m, ws, lds_limit);
     if (err != hipSuccess) return err;
     return launch_dual_newton(stream, Cr, Ci, sC_b, ldc, batch, m, omega, sO_b, ldo, Pf, Qf,
-                              iters, status, counters, ws, lds_limit);
+                              iters, status, counters, ws, lds_limit, caps);
 }
 
 size_t dual_lds_bytes(int m) {

@@ -15,6 +15,7 @@ void dual_uneven_kernel(const UTarget* __restrict__ tg)
     a.Cr = t.Cr; a.Ci = t.Ci; a.sC_b = 0; a.ldc = t.m; a.m = t.m; a.mp = t.mp;
     a.omega = t.omega; a.sO_b = 0; a.ldo = (int)t.ldo; a.Pf = t.Pf; a.Qf = t.Qf;
     a.iters = t.iters; a.status = t.status; a.counters = t.counters; a.ws = t.dual; a.v_in_lds = 0;
+    a.caps = Caps();
     switch (t.mp / 16) {
         case 1: dual_body<1>(a, 0); break;
         case 2: dual_body<2>(a, 0); break;

@@ -26,12 +26,12 @@ hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int
     if (SSN_EIGH_REPLAY) {
         // rotations only in the per-target kernel, then the eigenvectors replayed row by row
         const size_t lds = eigh_lds_doubles(m, false) * 8;
-        if (lds > lds_limit) return hipErrorInvalidValue;
+        if (lds > lds_limit) return LAUNCH_UNSUPPORTED;
         EighArgs e;
         e.Cr = Cr; e.sC_b = sC_b; e.ldc = ldc; e.m = m; e.mp = pad16(m); e.ws = ws; e.v_in_lds = 0;
         const int np = m / 2;
         const int blocks = (np * (np - 1) / 2 + JU - 1) / JU;
-        if (blocks > JB_MAX) return hipErrorInvalidValue;
+        if (blocks > JB_MAX) return LAUNCH_UNSUPPORTED;
         const bool small = blocks <= JB_BATCH;
         const void* fn = small ? (const void*)eigh_kernel<JB_BATCH, false, true> : (const void*)eigh_kernel<JB_MAX, false, true>;
         hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -45,7 +45,7 @@ hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int
     }
     const bool v_in_lds = eigh_lds_doubles(m, true) * 8 <= lds_limit;
     const size_t lds = eigh_lds_doubles(m, v_in_lds) * 8;
-    if (lds > lds_limit) return hipErrorInvalidValue;
+    if (lds > lds_limit) return LAUNCH_UNSUPPORTED;
     EighArgs e;
     e.Cr = Cr; e.sC_b = sC_b; e.ldc = ldc; e.m = m; e.mp = pad16(m); e.ws = ws;
     e.v_in_lds = v_in_lds ? 1 : 0;
@@ -53,7 +53,7 @@ hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int
     const int blocks = (np * (np - 1) / 2 + JU - 1) / JU;          // off-diagonal blocks per thread
     const int rows = (m + JU / np - 1) / (JU / np);                 // eigenvector rows per thread
     const bool small = blocks <= JB_BATCH && rows <= JV_SMALL;
-    if (blocks > JB_MAX || rows > 2 * JV_HALF) return hipErrorInvalidValue;
+    if (blocks > JB_MAX || rows > 2 * JV_HALF) return LAUNCH_UNSUPPORTED;
     const void* fn = small ? (v_in_lds ? (const void*)eigh_kernel<JB_BATCH, true> : (const void*)eigh_kernel<JB_BATCH, false>)
                            : (v_in_lds ? (const void*)eigh_kernel<JB_MAX, true> : (const void*)eigh_kernel<JB_MAX, false>);
     hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -44,8 +44,7 @@ namespace ssn {
 
 constexpr int DW = 8;                 // waves per target
 constexpr int DT = DW * WAVE;         // threads per target
-constexpr int MAX_NEWTON = 100, MAX_CG = 150, MAX_BACKTRACK = 40;   // dual_recommended_hvp.py:16
-constexpr double FEAS_TOL = 1e-8;                                    // dual_recommended_hvp.py:15
+constexpr int MAX_CG = 150;           // dual_recommended_hvp.py:16 (the adjoint's cap; the solver's come in Caps)
 constexpr int MAX_SWEEPS = 40;
 constexpr int NCOUNT = 12;            // counters per target
 
@@ -64,6 +63,7 @@ struct DualArgs {
     long long* iters; int* status; int* counters; // counters [b][12] (include/ssn.h)
     double* ws;                                   // [b][S_COUNT][mp*mp] + [b][mp] values
     int v_in_lds;
+    Caps caps;                                    // iteration caps and tolerance
 };
 
 template <int NT>
@@ -1159,7 +1159,8 @@ __device__ __forceinline__ void chain_term(int t, int J, int r, int q, const dou
 template <int NT, bool ADJ, class INIT>
 __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
                                           const double* Sr, const double* Si, const double* P, const double* QT,
-                                          const double* lam_s, double tol, double* Xg, double* Z0g, int& how)
+                                          const double* lam_s, double tol, double* Xg, double* Z0g, int& how,
+                                          int max_cg)
 {
     using K = ChainCfg<NT>;
     constexpr int MP = K::MP, LS = K::LS, UPT = K::UPT, UN = K::UN;
@@ -1202,7 +1203,7 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
     how = 2;
     if (ADJ && !(sqrt(rr0) > 0.0)) how = 1;
 #pragma unroll 1
-    for (int it = 0; it < MAX_CG && !(ADJ && how == 1); ++it) {
+    for (int it = 0; it < max_cg && !(ADJ && how == 1); ++it) {
         ++nh;
         d4 accZ[NT];
 #pragma unroll
@@ -1627,12 +1628,15 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
     t_setup = wall_clock64() - t_start;
 
     long long iters = 0;
-    int status = sweeps < 0 ? ST_EIGH : ST_OK;
+    const Caps caps = a.caps;
+    // max_newton = 0 leaves the Newton loop at once with the cap reached, as the reference's
+    // `for newton_step in range(MAX_NEWTON)` followed by the cap error does (:118, :191)
+    int status = sweeps < 0 ? ST_EIGH : (caps.max_newton <= 0 ? ST_NEWTON_CAP : ST_OK);
     int n_cg = 0, n_bt = 0, n_hvp = 0;
     const double sqrt_m = sqrt((double)m);
 
 #pragma unroll 1
-    for (int ns = 0; ns < MAX_NEWTON && status == ST_OK; ++ns) {
+    for (int ns = 0; ns < caps.max_newton && status == ST_OK; ++ns) {
         // -- sandwich S = N C N^H and the gradient J - Si (:120-129)
         double gg = 0.0;
         if (ns == 0) {
@@ -1676,7 +1680,7 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
         }
         const double gnorm = sqrt(wg_sum(c, gg));
         const double residual = gnorm * scale / sqrt_m;
-        if (!(residual >= FEAS_TOL / 2)) {
+        if (!(residual >= caps.feas_tol / 2)) {
             if (residual != residual) status = ST_NONFINITE;
             break;
         }
@@ -1694,7 +1698,7 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
             const unsigned long long t0 = wall_clock64();
             int how;
             const int nh = cg_chained<NT, false>(c, lds, [&](int i, int j) { return -G[i * MP + j]; },
-                                                 Sr, Si, P, QT, lam_s, tol, Dd, Z0, how);
+                                                 Sr, Si, P, QT, lam_s, tol, Dd, Z0, how, caps.max_cg);
             n_hvp += nh; n_cg += nh;
             t_cg += wall_clock64() - t0;
         } else {
@@ -1709,7 +1713,7 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
             const double tol = fmin(fmax(sqrt(residual), 1e-8), 0.1) * gnorm;
             const unsigned long long t0 = wall_clock64();
 #pragma unroll 1
-            for (int it = 0; it < MAX_CG; ++it) {
+            for (int it = 0; it < caps.max_cg; ++it) {
                 // Hp = hvp(P, Q, Sr, Si, p): U = p Sr, V = p Si, Z = P U - Q V, H = Z - Z^T
                 gemm<NT, false, false>(c, lds, Pp, Sr, [&](int i, int j, double v) { U[i * MP + j] = v; });
                 gemm<NT, false, false>(c, lds, Pp, Si, [&](int i, int j, double v) { Vv[i * MP + j] = v; });
@@ -1761,7 +1765,7 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
         bool accepted = false;
         const unsigned long long t1 = wall_clock64();
 #pragma unroll 1
-        for (int bt = 0; bt < MAX_BACKTRACK; ++bt) {
+        for (int bt = 0; bt < caps.max_backtrack; ++bt) {
             ++n_bt;
             double first = 0.0;
             // delta = step d; trial = skew(omega + delta) (:168-170). omega and d are exactly
@@ -1817,7 +1821,7 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
         }
         t_ls += wall_clock64() - t1;
         if (!accepted) { status = ST_NO_DESCENT; break; }
-        if (ns == MAX_NEWTON - 1) status = ST_NEWTON_CAP;
+        if (ns == caps.max_newton - 1) status = ST_NEWTON_CAP;
     }
     sync();
 

@@ -489,7 +489,7 @@ hipError_t launch_gram_partial(hipStream_t stream, const double* T, long long sT
     switch (nt) {
         SSN_GRAM_CASE(1) SSN_GRAM_CASE(2) SSN_GRAM_CASE(3) SSN_GRAM_CASE(4)
         SSN_GRAM_CASE(5) SSN_GRAM_CASE(6) SSN_GRAM_CASE(7) SSN_GRAM_CASE(8)
-        default: return hipErrorInvalidValue;
+        default: return LAUNCH_UNSUPPORTED;
     }
 #undef SSN_GRAM_CASE
     return hipGetLastError();
@@ -510,7 +510,7 @@ hipError_t launch_cross_gram(hipStream_t stream, const double* T, long long sT_b
     switch (nt) {
         SSN_CROSS_CASE(1) SSN_CROSS_CASE(2) SSN_CROSS_CASE(3) SSN_CROSS_CASE(4)
         SSN_CROSS_CASE(5) SSN_CROSS_CASE(6) SSN_CROSS_CASE(7) SSN_CROSS_CASE(8)
-        default: return hipErrorInvalidValue;
+        default: return LAUNCH_UNSUPPORTED;
     }
 #undef SSN_CROSS_CASE
     hipError_t e = hipGetLastError();

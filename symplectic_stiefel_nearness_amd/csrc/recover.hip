@@ -192,7 +192,7 @@ static hipError_t launch_stream(hipStream_t stream, const double* T, long long s
     switch (nt) {
         SSN_REC_CASE(1) SSN_REC_CASE(2) SSN_REC_CASE(3) SSN_REC_CASE(4)
         SSN_REC_CASE(5) SSN_REC_CASE(6) SSN_REC_CASE(7) SSN_REC_CASE(8)
-        default: return hipErrorInvalidValue;
+        default: return LAUNCH_UNSUPPORTED;
     }
 #undef SSN_REC_CASE
     return hipGetLastError();
@@ -312,13 +312,13 @@ void recover_lds_kernel(const double* __restrict__ T, long long sT_b, long long 
 }
 
 // Recovery straight from P and Q (no fragment array) when the LDS holds both: mp <= 96.
-// Returns hipErrorNotSupported otherwise (the caller builds fragments and uses launch_recover).
+// Returns LAUNCH_DECLINED otherwise (the caller builds fragments and uses launch_recover).
 hipError_t launch_recover_pq(hipStream_t stream, const double* T, long long sT_b, long long sT_r,
                              const double* P, const double* Q, int batch, int n, int m,
                              double* X, long long sX_b, long long sX_r, size_t lds_limit)
 {
     const int nt = pad16(m) / 16;
-    if (nt > 6) return hipErrorNotSupported;
+    if (nt > 6) return LAUNCH_DECLINED;
     const int ntiles = (n + 15) / 16;
     int wpt = (256 + batch - 1) / batch;                      // about one workgroup per CU
     const int most = (ntiles + RL_WAVES - 1) / RL_WAVES;
@@ -328,7 +328,7 @@ hipError_t launch_recover_pq(hipStream_t stream, const double* T, long long sT_b
 #define SSN_RL_CASE(NTV) \
     case NTV: { \
         const size_t lds = (size_t)recover_lds_doubles<NTV>() * 8; \
-        if (lds > lds_limit) return hipErrorNotSupported; \
+        if (lds > lds_limit) return LAUNCH_DECLINED; \
         hipError_t err = hipFuncSetAttribute((const void*)recover_lds_kernel<NTV>, \
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
         if (err != hipSuccess) return err; \
@@ -337,7 +337,7 @@ hipError_t launch_recover_pq(hipStream_t stream, const double* T, long long sT_b
         break; }
     switch (nt) {
         SSN_RL_CASE(1) SSN_RL_CASE(2) SSN_RL_CASE(3) SSN_RL_CASE(4) SSN_RL_CASE(5) SSN_RL_CASE(6)
-        default: return hipErrorNotSupported;
+        default: return LAUNCH_DECLINED;
     }
 #undef SSN_RL_CASE
     return hipGetLastError();

@@ -484,7 +484,7 @@ hipError_t launch_split_gram(hipStream_t stream, const double* T, long long sT_b
     switch (nt) {
         SSN_SG_CASE(1, 1) SSN_SG_CASE(2, 1) SSN_SG_CASE(3, 1) SSN_SG_CASE(4, 2)
         SSN_SG_CASE(5, 2) SSN_SG_CASE(6, 2) SSN_SG_CASE(7, 2) SSN_SG_CASE(8, 4)
-        default: return hipErrorInvalidValue;
+        default: return LAUNCH_UNSUPPORTED;
     }
 #undef SSN_SG_CASE
     return hipGetLastError();
@@ -508,7 +508,7 @@ hipError_t launch_split_recover(hipStream_t stream, const double* T, long long s
     switch (nt) {
         SSN_SR_CASE(1) SSN_SR_CASE(2) SSN_SR_CASE(3) SSN_SR_CASE(4)
         SSN_SR_CASE(5) SSN_SR_CASE(6) SSN_SR_CASE(7) SSN_SR_CASE(8)
-        default: return hipErrorInvalidValue;
+        default: return LAUNCH_UNSUPPORTED;
     }
 #undef SSN_SR_CASE
     return hipGetLastError();

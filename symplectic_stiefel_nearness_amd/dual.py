@@ -8,6 +8,7 @@ All arithmetic runs in libssn.so (HIP, gfx950); there is no CPU fallback.
 """
 from __future__ import annotations
 
+import ctypes
 import os
 import torch
 
@@ -79,8 +80,23 @@ def _workspace(nbytes, device):
     return torch.empty(max(int(nbytes), 8), dtype=torch.uint8, device=device)
 
 
-def _run_solve(T, split):
-    """One ssn_solve / ssn_solve_split call on device T; returns outputs, workspace and stats."""
+def _options(split, caps=None):
+    """ssn_options for one call: the module constants FEAS_TOL, MAX_NEWTON, MAX_CG and
+    MAX_BACKTRACK as they are at call time (the reference reads its module constants the same
+    way, dual_recommended_hvp.py:15-16), overridden by the non-None entries of `caps`
+    (keys max_newton, max_cg, max_backtrack, feas_tol)."""
+    caps = dict(caps or {})
+    bad = set(caps) - {"max_newton", "max_cg", "max_backtrack", "feas_tol"}
+    if bad:
+        raise TypeError(f"unknown solver caps: {sorted(bad)}")
+    get = lambda k, d: d if caps.get(k) is None else caps[k]
+    return _lib.SsnOptions(int(get("max_newton", MAX_NEWTON)), int(get("max_cg", MAX_CG)),
+                           int(get("max_backtrack", MAX_BACKTRACK)), 1 if split else 0,
+                           float(get("feas_tol", FEAS_TOL)))
+
+
+def _run_solve(T, split, caps=None):
+    """One ssn_solve_opts call on device T; returns outputs, workspace and stats."""
     lib = _require_ready()
     batch, n, m = _check_targets(T)
     dev = T.device
@@ -92,15 +108,15 @@ def _run_solve(T, split):
     status = torch.zeros(batch, dtype=torch.int32, device=dev)
     counters = torch.zeros((batch, 12), dtype=torch.int32, device=dev)
     ws = _workspace(lib.ssn_workspace_bytes(batch, n, m) if batch else 8, dev)
+    opts = _options(split, caps)
     if batch:
-        fn, name = (lib.ssn_solve_split, "ssn_solve_split") if split else (lib.ssn_solve, "ssn_solve")
         with torch.cuda.device(dev):
-            rc = fn(_stream(), T.data_ptr(), T.stride(0), T.stride(1), batch, n, m,
-                    X.data_ptr(), X.stride(0), X.stride(1),
-                    omega.data_ptr(), omega.stride(0), omega.stride(1),
-                    iters.data_ptr(), status.data_ptr(), counters.data_ptr(),
-                    ws.data_ptr(), ws.numel())
-        _lib.check(rc, name)
+            rc = lib.ssn_solve_opts(_stream(), ctypes.byref(opts), T.data_ptr(), T.stride(0), T.stride(1),
+                                    batch, n, m, X.data_ptr(), X.stride(0), X.stride(1),
+                                    omega.data_ptr(), omega.stride(0), omega.stride(1),
+                                    iters.data_ptr(), status.data_ptr(), counters.data_ptr(),
+                                    ws.data_ptr(), ws.numel())
+        _lib.check(rc, "ssn_solve_opts")
     return X, omega, iters, ws, dict(status=status, counters=counters)
 
 
@@ -112,8 +128,10 @@ def _solve_device_ws(T, check=True, split=False):
     return X, omega, iters, ws, stats
 
 
-def backward_device(T, G, ws, return_stats=False):
-    """dL/dT for the upstream gradient G = dL/dX, from the forward workspace `ws` of T's solve."""
+def backward_device(T, G, ws, return_stats=False, check=True):
+    """dL/dT for the upstream gradient G = dL/dX, from the forward workspace `ws` of T's solve.
+
+    check: raise RuntimeError when the adjoint's Hessian solve did not converge for a target."""
     lib = _require_ready()
     batch, n, m = _check_targets(T)
     assert G.shape == T.shape and G.dtype == torch.float64 and G.device == T.device
@@ -127,9 +145,21 @@ def backward_device(T, G, ws, return_stats=False):
                                   dT.data_ptr(), dT.stride(0), dT.stride(1), counters.data_ptr(),
                                   ws.data_ptr(), ws.numel())
         _lib.check(rc, "ssn_backward")
+        if check:
+            _raise_for_adjoint(counters)
     if return_stats:
         return dT, dict(counters=counters)
     return dT
+
+
+def _raise_for_adjoint(counters):
+    """The adjoint's Hessian solve per target (counters[:, 1]: 0 converged, 1 stopped at a
+    non-positive curvature or at the CG cap): an unconverged solve gives an inaccurate dL/dT,
+    so it raises as the forward raises on its statuses."""
+    bad = (counters[:, 1].cpu() != 0).nonzero().flatten().tolist()
+    if bad:
+        raise RuntimeError(f"adjoint Hessian solve did not converge for targets {bad[:8]}"
+                           f"{' ...' if len(bad) > 8 else ''}: dL/dT would be inaccurate")
 
 
 class _Projection(torch.autograd.Function):
@@ -170,7 +200,7 @@ def project(T, split=False):
 
 
 @torch.no_grad()
-def solve_device(T, return_stats=False, check=True, split=False):
+def solve_device(T, return_stats=False, check=True, split=False, **caps):
     """Device-resident solve: T is a float64 HIP tensor [batch, 2n, 2k]; outputs stay on the device.
 
     Returns (X, omega, iterations[, stats]) where stats = dict(status, counters[batch, 12]
@@ -178,9 +208,11 @@ def solve_device(T, return_stats=False, check=True, split=False):
     eigensolver, setup, CG, line search, whole kernel, 0).
     split=True computes the Gram and the recovery as split BF16 products with FP32
     accumulation reassembled to FP64 (every other step in FP64).
+    caps: max_newton, max_cg, max_backtrack, feas_tol; each defaults to the module constant
+    of that name (MAX_NEWTON, ...) at call time.
     """
     assert T.is_cuda, "solve_device expects a device tensor"
-    X, omega, iters, _, stats = _run_solve(T, split)
+    X, omega, iters, _, stats = _run_solve(T, split, caps)
     if check:
         _raise_for_status(stats["status"])
     if return_stats:
@@ -240,26 +272,26 @@ def solve_batches(Ts, in_flight=MAX_IN_FLIGHT, split=False, return_stats=False):
 
 
 @torch.no_grad()
-def solve(T, return_stats=False, split=False, pipeline=None):
+def solve(T, return_stats=False, split=False, pipeline=None, **caps):
     """(X, omega, iterations) for float64 targets T [batch, 2n, 2k] (dual_recommended_hvp.py:231-242).
 
     CPU input -> CPU outputs (H2D and D2H included, X pinned when >= 64 MiB as in
-    dual_recommended_hvp.py:204-205); device input -> device outputs. split: see solve_device.
-    pipeline: for CPU input, solve in chunks of targets with the host-to-device copies,
-    the solves and the device-to-host copies overlapped on separate streams (default: when
-    T is at least 64 MiB and holds two or more targets).
+    dual_recommended_hvp.py:204-205); device input -> device outputs. split, caps: see
+    solve_device. pipeline: for CPU input, solve in chunks of targets with the host-to-device
+    copies, the solves and the device-to-host copies overlapped on separate streams (default:
+    when T is at least 64 MiB and holds two or more targets).
     """
     _require_ready()
     batch = _check_targets(T)[0]
     if T.is_cuda:
-        return solve_device(T, return_stats=return_stats, split=split)
+        return solve_device(T, return_stats=return_stats, split=split, **caps)
     if pipeline is None:
         pipeline = batch >= 2 and T.numel() * T.element_size() >= PINNED_OUTPUT_MIN_BYTES
     if pipeline and batch >= 2:
-        return _solve_host_pipelined(T, return_stats, split)
+        return _solve_host_pipelined(T, return_stats, split, caps=caps)
     dev = torch.device("cuda", torch.cuda.current_device())
     Td = T.to(dev)
-    out = solve_device(Td, return_stats=return_stats, split=split)
+    out = solve_device(Td, return_stats=return_stats, split=split, **caps)
     X, omega, iters = out[:3]
     direct = X.numel() * X.element_size() >= PINNED_OUTPUT_MIN_BYTES
     Xh = torch.empty(X.shape, dtype=X.dtype, pin_memory=direct)
@@ -305,7 +337,7 @@ def _pipeline_streams(dev, count):
     return _pipe_streams[key]
 
 
-def _solve_host_pipelined(T, return_stats, split, chunks=None, streams=None, direct=None):
+def _solve_host_pipelined(T, return_stats, split, chunks=None, streams=None, direct=None, caps=None):
     """CPU-in / CPU-out solve with the transfers overlapped (dual_recommended_hvp.py:79-96, :194-228).
 
     The reference streams rows through pinned staging buffers (ROW_CHUNK rows) around one
@@ -339,12 +371,13 @@ def _solve_host_pipelined(T, return_stats, split, chunks=None, streams=None, dir
     if direct is None:
         direct = threads < 4
     contiguous = T.is_contiguous()
+    # ssn_workspace_bytes is non-decreasing in the batch, so the biggest chunk's size serves all
     biggest = max(b1 - b0 for b0, b1 in bounds)
     wsb = lib.ssn_workspace_bytes(biggest, n, m)
     ws = [_workspace(wsb, dev) for _ in comp]          # one per solve stream, reused in order
     for s_ in (h2d, d2h, *comp):
         s_.wait_stream(cur)                            # buffers and inputs are ready
-    fn, name = (lib.ssn_solve_split, "ssn_solve_split") if split else (lib.ssn_solve, "ssn_solve")
+    opts = _options(split, caps)
     for c, (b0, b1) in enumerate(bounds):
         if direct:
             with torch.cuda.stream(h2d):
@@ -365,12 +398,12 @@ def _solve_host_pipelined(T, return_stats, split, chunks=None, streams=None, dir
         s_.wait_event(arrived)
         with torch.cuda.stream(s_):
             Tc, Xc = Td[b0:b1], Xd[b0:b1]
-            rc = fn(s_.cuda_stream, Tc.data_ptr(), Tc.stride(0), Tc.stride(1), b1 - b0, n, m,
-                    Xc.data_ptr(), Xc.stride(0), Xc.stride(1),
-                    omega[b0:b1].data_ptr(), omega.stride(0), omega.stride(1),
-                    iters[b0:b1].data_ptr(), status[b0:b1].data_ptr(), counters[b0:b1].data_ptr(),
-                    ws[k].data_ptr(), ws[k].numel())
-            _lib.check(rc, name)
+            rc = lib.ssn_solve_opts(s_.cuda_stream, ctypes.byref(opts), Tc.data_ptr(), Tc.stride(0),
+                                    Tc.stride(1), b1 - b0, n, m, Xc.data_ptr(), Xc.stride(0), Xc.stride(1),
+                                    omega[b0:b1].data_ptr(), omega.stride(0), omega.stride(1),
+                                    iters[b0:b1].data_ptr(), status[b0:b1].data_ptr(),
+                                    counters[b0:b1].data_ptr(), ws[k].data_ptr(), ws[k].numel())
+            _lib.check(rc, "ssn_solve_opts")
             solved = s_.record_event()
         d2h.wait_event(solved)
         with torch.cuda.stream(d2h):

@@ -33,6 +33,10 @@ class UnevenSolver:
     shapes: list of (n, k); targets are [2n, 2k] float64. run(targets) copies the targets into
     the solver's input buffers and returns [(X, omega, iterations)], views of the solver's
     output buffers (overwritten by the next run).
+    inputs: optional device tensors used AS the input buffers (no copy, so a planned or captured
+    solve reads them in place). They are aliased, not owned: run(targets) with new targets
+    copies those into these tensors, overwriting the caller's data; run() without targets
+    solves whatever they hold at that moment.
     """
 
     def __init__(self, shapes, device=None, tuning=None, capture=False, inputs=None):

@@ -70,3 +70,47 @@ def test_host_copy(lib):
         assert np.array_equal(dst, src)
     assert lib.ssn_host_copy(None, None, 0, 4) == 0
     assert lib.ssn_host_copy(None, ctypes.c_void_p(1), 8, 4) == -1
+
+
+def test_workspace_bytes_non_decreasing_in_batch(lib):
+    """A workspace sized for a batch serves every smaller batch of the same shape: the Gram's chunk
+    count falls as the batch grows, and the partial slabs are sized for the worst smaller batch.
+    Also covers every chunk split of the CPU-in pipeline (sized by its biggest chunk)."""
+    L = _lib.load()
+    for n in (100, 2816, 4096, 8192):
+        for m in (16, 96):
+            sizes = [L.ssn_workspace_bytes(b, n, m) for b in range(1, 601)]
+            assert all(a <= b for a, b in zip(sizes, sizes[1:])), (n, m)
+            for batch in range(2, 401):
+                nch = min(4, batch)
+                bounds = [(batch * c // nch, batch * (c + 1) // nch) for c in range(nch)]
+                biggest = max(b1 - b0 for b0, b1 in bounds)
+                need = max(sizes[b1 - b0 - 1] for b0, b1 in bounds)
+                assert sizes[biggest - 1] >= need, (n, m, batch)
+
+
+def test_default_options_are_the_reference_constants(lib):
+    L = _lib.load()
+    o = _lib.SsnOptions()
+    L.ssn_default_options(ctypes.byref(o))
+    assert (o.max_newton, o.max_cg, o.max_backtrack, o.split, o.feas_tol) == (100, 150, 40, 0, 1e-8)
+    from symplectic_stiefel_nearness_amd import dual
+    d = dual._options(False)
+    assert (d.max_newton, d.max_cg, d.max_backtrack, d.split, d.feas_tol) == (100, 150, 40, 0, 1e-8)
+    assert dual._options(True, dict(max_cg=0)).max_cg == 0 and dual._options(True).split == 1
+
+
+def test_solve_opts_rejects_bad_options_before_any_launch(lib):
+    """Argument errors come back before any HIP call (so this runs without a GPU)."""
+    L = _lib.load()
+    args = [None, 0, 0, 1, 16, 16, None, 0, 0, None, 0, 16, None, None, None, None, 0]
+    assert L.ssn_solve_opts(None, None, *args) == _lib.SSN_ERR_ARGS
+    for field, value in (("max_newton", -1), ("max_cg", -1), ("max_backtrack", 1101), ("split", 2),
+                         ("feas_tol", float("nan")), ("feas_tol", -1.0), ("feas_tol", float("inf"))):
+        o = _lib.SsnOptions()
+        L.ssn_default_options(ctypes.byref(o))
+        setattr(o, field, value)
+        assert L.ssn_solve_opts(None, ctypes.byref(o), *args) == _lib.SSN_ERR_ARGS, field
+    o = _lib.SsnOptions()
+    L.ssn_default_options(ctypes.byref(o))
+    assert L.ssn_solve_opts(None, ctypes.byref(o), *args) == _lib.SSN_ERR_ARGS     # null T

@@ -67,3 +67,4 @@ def test_summary_rejects_missing_calls(tmp_path):
     path.write_text("\n".join(lines[:-1]) + "\n")
     with pytest.raises(ValueError):
         experiment.summarize(tmp_path, p)
+

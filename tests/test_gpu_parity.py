@@ -121,9 +121,10 @@ def test_solve_order128(ssn):
     _solve_and_compare(ssn, T)
 
 
-@pytest.mark.parametrize("n,k", [(3, 1), (40, 3), (150, 10), (500, 25)])
+@pytest.mark.parametrize("n,k", [(3, 1), (40, 3), (150, 10), (500, 25), (1200, 36), (800, 52)])
 def test_solve_unpadded_orders(ssn, n, k):
-    """Orders m = 2k that are not multiples of 16 exercise the zero padding."""
+    """Orders m = 2k that are not multiples of 16 exercise the zero padding; m = 72 and m = 104
+    are the odd tile orders NT = 5 and NT = 7 (staged CG, sandwich and Cholesky blocks)."""
     say("solve padded", n, k)
     T = inputs.make_targets(n, k, 2, seeds=[11 + n, 12 + n])
     _solve_and_compare(ssn, T)
