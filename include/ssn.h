@@ -38,7 +38,8 @@ extern "C" {
 #define SSN_ST_EIGH 4            /* Jacobi eigensolver did not converge                             */
 
 /* Replaces initialize_kernels() (src/symplectic_projection/__init__.py:10-19): there is
- * nothing to JIT-compile; it caches the current device's LDS limit. Idempotent. */
+ * nothing to JIT-compile; it caches the current device's LDS limit and the device address of
+ * the static scratch of ssn_solve_static on that device. Idempotent. */
 int ssn_init(void);
 
 const char* ssn_version(void);
@@ -129,6 +130,21 @@ int ssn_solve_opts(void* stream, const ssn_options* opts, const double* T, long 
                    int batch, int n, int m, double* X, long long sX_b, long long sX_r, double* omega,
                    long long sO_b, int ldo, long long* iters, int* status, int* counters, void* ws,
                    size_t ws_bytes);
+
+/* Allocation-free form of ssn_solve (same arguments without ws / ws_bytes, same results bit for
+ * bit). No counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * workspace is a static device array of SSN_STATIC_SCRATCH_BYTES inside the library, one per GPU,
+ * held from ssn_init (or from the first call on that GPU) until the process ends; the call
+ * allocates and frees nothing. Returns SSN_ERR_WORKSPACE when ssn_workspace_bytes(batch, n, m)
+ * exceeds SSN_STATIC_SCRATCH_BYTES (e.g. more than 16 targets of 8192 x 96, or more than 2 of
+ * 16384 x 128). All calls on one GPU share that GPU's array and it keeps the last call's
+ * contents (the solve needs none of it cleared): calls that overlap in time on one GPU (several
+ * streams or host threads) race on it, and the caller must serialise them. */
+#define SSN_STATIC_SCRATCH_BYTES (128ull << 20)
+int ssn_solve_static(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+                     double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
+                     long long* iters, int* status, int* counters);
 
 /* Split-precision forms of ssn_gram, ssn_recover and ssn_solve (same arguments and workspace).
  * No reference counterpart (the reference's Gram and recovery are FP64 only,

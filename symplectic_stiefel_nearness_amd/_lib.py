@@ -16,6 +16,7 @@ SSN_ERR_HIP = -2
 SSN_ERR_WORKSPACE = -3
 SSN_ERR_UNSUPPORTED = -4
 SSN_ERR_INIT = -5
+SSN_STATIC_SCRATCH_BYTES = 128 << 20   # include/ssn.h: the static workspace of ssn_solve_static
 
 ST_OK, ST_NO_DESCENT, ST_NEWTON_CAP, ST_NONFINITE, ST_EIGH = 0, 1, 2, 3, 4
 
@@ -62,6 +63,7 @@ SIGNATURES = {
     "ssn_dual_newton": (_I, [_P, _P, _P, _L, _I, _I, _I, _P, _L, _I, _P, _P, _P, _P, _P, _P, _S]),
     "ssn_recover": (_I, [_P, _P, _L, _L, _P, _P, _I, _I, _I, _P, _L, _L, _P, _S]),
     "ssn_solve": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _L, _I, _P, _P, _P, _P, _S]),
+    "ssn_solve_static": (_I, [_P, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _L, _I, _P, _P, _P]),
     "ssn_default_options": (None, [_OP]),
     "ssn_solve_opts": (_I, [_P, _OP, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _L, _I, _P, _P, _P, _P, _S]),
     "ssn_backward": (_I, [_P, _P, _L, _L, _P, _L, _L, _I, _I, _I, _P, _L, _L, _P, _P, _S]),

@@ -1,7 +1,8 @@
 // extern "C" entry points of libssn.so (declared in include/ssn.h).
 // Plain pointers and sizes only; every device pointer is caller-owned, every call
 // is asynchronous on the caller's stream, and no entry point throws or allocates:
-// scratch comes in through the `ws` argument, sized by ssn_workspace_bytes().
+// scratch comes in through the `ws` argument, sized by ssn_workspace_bytes(), except in
+// ssn_solve_static, whose scratch is the library's static device array.
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <algorithm>
@@ -62,12 +63,31 @@ using namespace ssn;
 #define SLOT_KR 10
 #define SLOT_KI 11
 
+// Workspace of ssn_solve_static: a static device array, one copy per GPU (each GPU holds its own
+// at its own address). In host code `scratch` names a host-side stand-in, not device memory: the
+// library only ever uses the device address that hipGetSymbolAddress returns for the current
+// device (g_scratch below), never the name itself.
+static __device__ __align__(256) double scratch[SSN_STATIC_SCRATCH_BYTES / sizeof(double)];
+
 namespace {
 
 constexpr int MAX_DEVICES = 64;
 size_t g_lds_limit[MAX_DEVICES];
 bool g_ready[MAX_DEVICES];
+void* g_scratch[MAX_DEVICES];          // device address of `scratch` on each device, once looked up
 hipError_t g_last_hip = hipSuccess;
+
+// Device address of the static scratch on device `dev` (current), looked up once per device.
+int static_scratch(int dev, void** out) {
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) { g_last_hip = e != hipSuccess ? e : hipErrorInvalidSymbol; return SSN_ERR_HIP; }
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return SSN_OK;
+}
 
 int current_device_ready(size_t* lds) {
     int dev = 0;
@@ -134,7 +154,14 @@ Layout layout(int batch, int n, int m) {
 
 extern "C" {
 
-int ssn_init(void) { return current_device_ready(nullptr); }
+int ssn_init(void) {
+    const int rc = current_device_ready(nullptr);
+    if (rc) return rc;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return SSN_ERR_INIT;
+    void* p = nullptr;
+    return static_scratch(dev, &p);
+}
 
 const char* ssn_version(void) { return "ssn-gfx950 0.1.0"; }
 
@@ -327,6 +354,24 @@ int ssn_solve_split(void* stream, const double* T, long long sT_b, long long sT_
 {
     return solve_impl(true, Caps(), stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo, iters,
                       status, counters, ws, ws_bytes);
+}
+
+int ssn_solve_static(void* stream, const double* T, long long sT_b, long long sT_r, int batch, int n, int m,
+                     double* X, long long sX_b, long long sX_r, double* omega, long long sO_b, int ldo,
+                     long long* iters, int* status, int* counters)
+{
+    if (!shape_ok(batch, n, m) || !T || !X || !omega || !iters || !status || ldo < m) return SSN_ERR_ARGS;
+    if (batch == 0) return SSN_OK;
+    if (layout(batch, n, m).total * sizeof(double) > SSN_STATIC_SCRATCH_BYTES) return SSN_ERR_WORKSPACE;
+    int rc = current_device_ready(nullptr);
+    if (rc) return rc;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return SSN_ERR_INIT;
+    void* ws = nullptr;
+    rc = static_scratch(dev, &ws);
+    if (rc) return rc;
+    return solve_impl(false, Caps(), stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo, iters,
+                      status, counters, ws, SSN_STATIC_SCRATCH_BYTES);
 }
 
 void ssn_default_options(ssn_options* opts)
