@@ -279,31 +279,37 @@ def main():
     log(f"fwd+bwd: fp64 {fb[False]:.3f} ms, split forward {fb[True]:.3f} ms (backward alone {bwd_ms:.3f} ms, "
         f"adjoint CG {adj_it:.1f} it)")
 
-    # uneven batch: 128 seeded targets of mixed (n, k), eager launches and graph replays
+    # uneven batch: 128 seeded targets of mixed (n, k) with autotuned Gram / recovery tiles:
+    # the two-plan solve (largest order and the rest on two streams) eager and as graph
+    # replays, beside the single-plan eager solve with the default tiles
     uneven = None
     if not args.no_uneven:
         Tu = [t.to(dev) for t in inputs.make_uneven(128)]
-        solver = ssn.UnevenSolver([(t.shape[0] // 2, t.shape[1] // 2) for t in Tu], inputs=Tu)
-        solver.run(check=False)
-        torch.cuda.synchronize()
-        t5 = time.perf_counter()
-        for _ in range(3):
+        shapes_u = [(t.shape[0] // 2, t.shape[1] // 2) for t in Tu]
+
+        def timed_runs(solver, reps=3):
             solver.run(check=False)
-        torch.cuda.synchronize()
-        eager_ms = (time.perf_counter() - t5) * 1e3 / 3
-        solver_g = ssn.UnevenSolver([(t.shape[0] // 2, t.shape[1] // 2) for t in Tu], inputs=Tu, capture=True)
-        solver_g.run(check=False)
-        torch.cuda.synchronize()
-        t6 = time.perf_counter()
-        for _ in range(3):
-            solver_g.run(check=False)
-        torch.cuda.synchronize()
-        graph_ms = (time.perf_counter() - t6) * 1e3 / 3
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                solver.run(check=False)
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) * 1e3 / reps
+
+        single_ms = timed_runs(ssn.UnevenSolver(shapes_u, inputs=Tu, groups=1))
+        t_tune = time.perf_counter()
+        table = ssn.autotune(shapes_u)
+        tune_s = time.perf_counter() - t_tune
+        eager_ms = timed_runs(ssn.UnevenSolver(shapes_u, inputs=Tu, tuning=table))
+        solver_g = ssn.UnevenSolver(shapes_u, inputs=Tu, tuning=table, capture=True)
+        graph_ms = timed_runs(solver_g)
         ok = bool((solver_g.status == 0).all().item())
-        uneven = {"targets": 128, "eager_ms": eager_ms, "graph_ms": graph_ms,
-                  "solves_per_s": 128 / (graph_ms * 1e-3), "all_status_ok": ok}
-        log(f"uneven 128: eager {eager_ms:.2f} ms, graph {graph_ms:.2f} ms")
-        del Tu, solver, solver_g
+        uneven = {"targets": 128, "tiles": "autotuned", "plans": len(solver_g.groups),
+                  "eager_ms": eager_ms, "graph_ms": graph_ms, "single_plan_default_tiles_eager_ms": single_ms,
+                  "solves_per_s": 128 / (graph_ms * 1e-3), "autotune_s": tune_s, "all_status_ok": ok}
+        log(f"uneven 128: single plan {single_ms:.2f} ms; two plans, autotuned: eager {eager_ms:.2f} ms, "
+            f"graph {graph_ms:.2f} ms (autotune {tune_s:.1f} s)")
+        del Tu, solver_g
 
     # reference protocol boundary: CPU tensor in -> CPU tensors out (PCIe included), with the
     # chunked transfer pipeline (default for this size) and without it

@@ -201,6 +201,16 @@ int ssn_uneven_plan(const ssn_target* targets, int count, void* ws, size_t ws_by
                     size_t plan_bytes);
 int ssn_solve_uneven(void* stream, const void* plan, void* ws, size_t ws_bytes);
 
+/* The stages of ssn_solve_uneven selected by `stages` (an OR of the flags below), in stage order,
+ * for callers that time or overlap them: Gram (partials and their reduction into Cr, Ci), dual
+ * (eigensolver and Newton iteration, from Cr, Ci), recovery (X from P, Q). A stage reads what the
+ * earlier stages left in `ws`; SSN_STAGE_ALL is ssn_solve_uneven. */
+#define SSN_STAGE_GRAM 1
+#define SSN_STAGE_DUAL 2
+#define SSN_STAGE_RECOVER 4
+#define SSN_STAGE_ALL 7
+int ssn_solve_uneven_stages(void* stream, const void* plan, void* ws, size_t ws_bytes, int stages);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,7 @@ SSN_ERR_HIP = -2
 SSN_ERR_WORKSPACE = -3
 SSN_ERR_UNSUPPORTED = -4
 SSN_ERR_INIT = -5
+SSN_STAGE_GRAM, SSN_STAGE_DUAL, SSN_STAGE_RECOVER, SSN_STAGE_ALL = 1, 2, 4, 7   # ssn_solve_uneven_stages
 SSN_STATIC_SCRATCH_BYTES = 128 << 20   # include/ssn.h: the static workspace of ssn_solve_static
 
 ST_OK, ST_NO_DESCENT, ST_NEWTON_CAP, ST_NONFINITE, ST_EIGH = 0, 1, 2, 3, 4
@@ -74,6 +75,7 @@ SIGNATURES = {
     "ssn_uneven_plan_bytes": (_S, [_TP, _I]),
     "ssn_uneven_plan": (_I, [_TP, _I, _P, _S, _P, _S]),
     "ssn_solve_uneven": (_I, [_P, _P, _P, _S]),
+    "ssn_solve_uneven_stages": (_I, [_P, _P, _P, _S, _I]),
 }
 
 _lib = None

@@ -1,8 +1,9 @@
 """Autotuner of the uneven batch's tile shapes.
 
-For every distinct (n, m) it times the uneven solve of a small homogeneous batch of that
-shape under each candidate (Gram rows per work item, recovery rows per work item) and keeps
-the fastest. Results can be kept in a JSON cache (path argument or SSN_TUNING_CACHE).
+For every distinct (n, m) it times the Gram and recovery stages (the stages the tiles change;
+ssn_solve_uneven_stages) of a small homogeneous batch of that shape under each candidate (Gram
+rows per work item, recovery rows per work item) and keeps the fastest. Results can be kept in
+a JSON cache (path argument or SSN_TUNING_CACHE).
 """
 from __future__ import annotations
 
@@ -13,6 +14,7 @@ from pathlib import Path
 
 import torch
 
+from . import _lib
 from .uneven import UnevenSolver
 
 GRAM_ROWS_CANDIDATES = (256, 512, 1024, 2048)
@@ -57,13 +59,15 @@ def autotune(shapes, batch=4, reps=3, gram_rows=GRAM_ROWS_CANDIDATES, rec_rows=R
                 continue
             for rr in rec_rows:
                 tuning = {(n, m): (cr, rr)}
-                solver = UnevenSolver([(n, k)] * batch, device=dev, tuning=tuning, inputs=T)
-                solver.run(check=False)
+                solver = UnevenSolver([(n, k)] * batch, device=dev, tuning=tuning, inputs=T, groups=1)
+                solver.run(check=False)              # leaves P, Q for the recovery stage
+                stages = _lib.SSN_STAGE_GRAM | _lib.SSN_STAGE_RECOVER
+                solver.run(check=False, stages=stages)
                 torch.cuda.synchronize()
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record()
                 for _ in range(reps):
-                    solver.run(check=False)
+                    solver.run(check=False, stages=stages)
                 e1.record()
                 torch.cuda.synchronize()
                 ms = e0.elapsed_time(e1) / reps

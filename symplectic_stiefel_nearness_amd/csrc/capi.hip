@@ -42,7 +42,7 @@ long long split_fragment_doubles(int m);
 hipError_t launch_split_recover(hipStream_t, const double*, long long, long long, const double*, const double*,
                                 double*, int, int, int, double*, long long, long long);
 hipError_t launch_gram_uneven(hipStream_t, const UTarget*, const UItem*, const int*, int);
-hipError_t launch_dual_uneven(hipStream_t, const UTarget*, int, size_t, size_t);
+hipError_t launch_dual_uneven(hipStream_t, const UTarget*, int, size_t, size_t, int);
 hipError_t launch_recover_uneven(hipStream_t, const UTarget*, const UItem*, int, int, int);
 void eigh_variant(int m, size_t lds_limit, int* small, int* vlds, size_t* lds_bytes);
 size_t dual_lds_bytes(int m);
@@ -603,7 +603,12 @@ int ssn_uneven_plan(const ssn_target* targets, int count, void* ws, size_t ws_by
 
 int ssn_solve_uneven(void* stream, const void* plan, void* ws, size_t ws_bytes)
 {
-    if (!plan || !ws) return SSN_ERR_ARGS;
+    return ssn_solve_uneven_stages(stream, plan, ws, ws_bytes, SSN_STAGE_ALL);
+}
+
+int ssn_solve_uneven_stages(void* stream, const void* plan, void* ws, size_t ws_bytes, int stages)
+{
+    if (!plan || !ws || stages < 0 || stages > SSN_STAGE_ALL) return SSN_ERR_ARGS;
     const UPlanHeader* h = (const UPlanHeader*)plan;
     if (h->count <= 0 || ws_bytes < (size_t)h->ws_bytes) return SSN_ERR_WORKSPACE;
     int rc = current_device_ready(nullptr);
@@ -613,11 +618,20 @@ int ssn_solve_uneven(void* stream, const void* plan, void* ws, size_t ws_bytes)
     const UTarget* tg = (const UTarget*)(dp + h->targets_off);
     const UItem* gi = (const UItem*)(dp + h->gram_off);
     const UItem* ri = (const UItem*)(dp + h->rec_off);
-    rc = hip_status(launch_gram_uneven(s, tg, gi, h->gram_begin, h->count));
-    if (rc) return rc;
-    rc = hip_status(launch_dual_uneven(s, tg, h->count, (size_t)h->eigh_lds, (size_t)h->dual_lds));
-    if (rc) return rc;
-    return hip_status(launch_recover_uneven(s, tg, ri, h->rec_items, h->count, h->max_mp));
+    if (stages & SSN_STAGE_GRAM) {
+        rc = hip_status(launch_gram_uneven(s, tg, gi, h->gram_begin, h->count));
+        if (rc) return rc;
+    }
+    if (stages & SSN_STAGE_DUAL) {
+        int single_nt = 0, orders = 0;       // one order in the plan: its specialised launch
+        for (int k = 1; k <= 8; ++k)
+            if (h->gram_begin[k] > h->gram_begin[k - 1]) { single_nt = k; ++orders; }
+        rc = hip_status(launch_dual_uneven(s, tg, h->count, (size_t)h->eigh_lds, (size_t)h->dual_lds,
+                                           orders == 1 ? single_nt : 0));
+        if (rc) return rc;
+    }
+    if (stages & SSN_STAGE_RECOVER) return hip_status(launch_recover_uneven(s, tg, ri, h->rec_items, h->count, h->max_mp));
+    return SSN_OK;
 }
 
 }  // extern "C"

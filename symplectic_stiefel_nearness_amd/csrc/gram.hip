@@ -200,45 +200,19 @@ void gram_partial_kernel(const double* __restrict__ T, long long sT_b, long long
     gram_partial_body<NT>(T, sT_b, sT_r, n, m, chunk_rows, chunks, partial, blockIdx.x, blockIdx.y, stage);
 }
 
-// Deterministic second stage: Cr, Ci [batch][m][m] (leading dimension ldc) from the
-// chunk partials, summed in ascending chunk order. Cr is mirrored from its upper
-// triangle, Ci = S - S^T from the A^T D sums, so symmetry / skewness is exact.
-__device__ __forceinline__ void gram_reduce_element(const double* __restrict__ p, int chunks, int m, int mp,
-                                                    double* __restrict__ Crb, double* __restrict__ Cib, int ldc, int e)
-{
-    const int i = e / m, j = e - (e / m) * m;
-    const int lo = min(i, j), hi = max(i, j);
-    double cr = 0.0, ad = 0.0, da = 0.0;
-    for (int c = 0; c < chunks; ++c) {
-        const double* pc = p + (long long)c * 2 * mp * mp;
-        cr += pc[lo * mp + hi];
-        ad += pc[mp * mp + lo * mp + hi];
-        da += pc[mp * mp + hi * mp + lo];
-    }
-    const double ci_upper = ad - da;   // Ci[lo][hi]
-    Crb[(long long)i * ldc + j] = cr;
-    Cib[(long long)i * ldc + j] = (i == j) ? 0.0 : (i < j ? ci_upper : -ci_upper);
-}
-
-// Same sums as gram_reduce_element, one workgroup of 256 threads per (tile pair I <= J, target):
-// thread (r, c) sums element (r, c) of the upper tile (I, J) of slot 0 and of tiles (I, J),
-// (J, I) of slot 1 over the chunks in ascending order, so every chunk read is a contiguous
-// 16-double row; the transposed operands are then exchanged through LDS. Each output element
-// is the same expression of the same sums as in gram_reduce_element (bitwise equal results).
-__global__ __launch_bounds__(256)
-void gram_reduce_tiles_kernel(const double* __restrict__ partial, int chunks, int m, int mp,
-                              double* __restrict__ Cr, double* __restrict__ Ci,
-                              long long sC_b, int ldc)
+// Deterministic second stage: Cr, Ci [batch][m][m] (leading dimension ldc) from the chunk
+// partials, summed in ascending chunk order, one workgroup of 256 threads per (tile pair I <= J,
+// target): thread (r, c) sums element (r, c) of the upper tile (I, J) of slot 0 (Cr) and of
+// tiles (I, J), (J, I) of slot 1 (A^T D) over the chunks, so every chunk read is a contiguous
+// 16-double row; the transposed operands are then exchanged through LDS. Cr is mirrored from its
+// upper triangle and Ci = S - S^T from the A^T D sums, so symmetry and skewness are exact.
+// l0 .. l2 are LDS tiles.
+__device__ __forceinline__ void gram_reduce_tile_body(const double* __restrict__ p, int chunks, int m, int mp,
+                                                      double* __restrict__ Crb, double* __restrict__ Cib,
+                                                      int ldc, int I, int J, double* l0, double* l1, double* l2)
 {
     constexpr int LS = 17;
-    __shared__ double l0[16 * LS], l1[16 * LS], l2[16 * LS];
-    const int b = blockIdx.y;
-    int I = 0, rem = blockIdx.x;                 // blockIdx.x -> (I, J), I <= J, row-major upper
-    const int nt = mp / 16;
-    while (rem >= nt - I) { rem -= nt - I; ++I; }
-    const int J = I + rem;
     const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
-    const double* p = partial + (long long)b * chunks * 2 * mp * mp;
     const int oij = (I * 16 + r) * mp + J * 16 + c;     // element (r, c) of tile (I, J)
     const int oji = (J * 16 + r) * mp + I * 16 + c;     // element (r, c) of tile (J, I)
     double cr = 0.0, sij = 0.0, sji = 0.0;
@@ -252,8 +226,6 @@ void gram_reduce_tiles_kernel(const double* __restrict__ partial, int chunks, in
     l1[r * LS + c] = sij;
     l2[r * LS + c] = sji;
     __syncthreads();
-    double* Crb = Cr + (long long)b * sC_b;
-    double* Cib = Ci + (long long)b * sC_b;
     {   // tile (I, J): i = 16I + r, j = 16J + c
         const int i = I * 16 + r, j = J * 16 + c;
         double vr, vi;
@@ -281,6 +253,27 @@ void gram_reduce_tiles_kernel(const double* __restrict__ partial, int chunks, in
     }
 }
 
+// blockIdx.x -> upper tile pair (I, J), I <= J, row-major over the nt x nt tiles
+__device__ __forceinline__ void tile_pair(int idx, int nt, int& I, int& J) {
+    int rem = idx;
+    I = 0;
+    while (rem >= nt - I) { rem -= nt - I; ++I; }
+    J = I + rem;
+}
+
+__global__ __launch_bounds__(256)
+void gram_reduce_tiles_kernel(const double* __restrict__ partial, int chunks, int m, int mp,
+                              double* __restrict__ Cr, double* __restrict__ Ci,
+                              long long sC_b, int ldc)
+{
+    __shared__ double l0[16 * 17], l1[16 * 17], l2[16 * 17];
+    const int b = blockIdx.y;
+    int I, J;
+    tile_pair(blockIdx.x, mp / 16, I, J);
+    gram_reduce_tile_body(partial + (long long)b * chunks * 2 * mp * mp, chunks, m, mp,
+                          Cr + (long long)b * sC_b, Ci + (long long)b * sC_b, ldc, I, J, l0, l1, l2);
+}
+
 // ---------------------------------------------------------------- uneven batches
 // Persistent grid over the (target, chunk) items of one order of a plan (largest first).
 template <int NT>
@@ -296,12 +289,18 @@ void gram_uneven_kernel(const UTarget* __restrict__ tg, const UItem* __restrict_
     }
 }
 
-// One workgroup per target: Cr, Ci (ld m) from its chunk partials.
-__global__ void gram_reduce_uneven_kernel(const UTarget* __restrict__ tg)
+// Cr, Ci (ld m) of every target of a plan from its chunk partials: blockIdx.y = target,
+// blockIdx.x = tile pair (grid sized for the largest order; pairs past a target's return at once).
+__global__ __launch_bounds__(256)
+void gram_reduce_uneven_kernel(const UTarget* __restrict__ tg)
 {
-    const UTarget& t = tg[blockIdx.x];
-    for (int e = threadIdx.x; e < t.m * t.m; e += blockDim.x)
-        gram_reduce_element(t.partial, t.chunks, t.m, t.mp, t.Cr, t.Ci, t.m, e);
+    __shared__ double l0[16 * 17], l1[16 * 17], l2[16 * 17];
+    const UTarget& t = tg[blockIdx.y];
+    const int nt = t.mp / 16;
+    if ((int)blockIdx.x >= nt * (nt + 1) / 2) return;
+    int I, J;
+    tile_pair(blockIdx.x, nt, I, J);
+    gram_reduce_tile_body(t.partial, t.chunks, t.m, t.mp, t.Cr, t.Ci, t.m, I, J, l0, l1, l2);
 }
 
 // ---------------------------------------------------------------- cross Gram (backward)
@@ -533,6 +532,9 @@ hipError_t launch_gram_reduce(hipStream_t stream, const double* partial, int bat
 hipError_t launch_gram_uneven(hipStream_t stream, const UTarget* tg, const UItem* items, const int* begin,
                               int count)
 {
+    int max_pairs = 1;                  // tile pairs of the largest order present
+    for (int nt = 1; nt <= 8; ++nt)
+        if (begin[nt] > begin[nt - 1]) max_pairs = nt * (nt + 1) / 2;
     for (int nt = 1; nt <= 8; ++nt) {
         const int first = begin[nt - 1], nitems = begin[nt] - first;
         if (nitems <= 0) continue;
@@ -548,7 +550,7 @@ hipError_t launch_gram_uneven(hipStream_t stream, const UTarget* tg, const UItem
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(gram_reduce_uneven_kernel, dim3(count), dim3(256), 0, stream, tg);
+    hipLaunchKernelGGL(gram_reduce_uneven_kernel, dim3(max_pairs, count), dim3(256), 0, stream, tg);
     return hipGetLastError();
 }
 

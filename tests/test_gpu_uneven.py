@@ -50,12 +50,15 @@ def test_uneven_tiles_do_not_change_results_beyond_rounding(ssn):
 
 
 def test_uneven_config3_full(ssn):
-    """The 128-target uneven workload: acceptance on every target (device FP64 checker, then
-    the CPU oracle checker on every 16th), oracle parity on the smallest targets."""
+    """The 128-target uneven workload with the autotuner's tiles (two plans on two streams):
+    acceptance on every target (device FP64 checker, then the CPU oracle checker on every 16th),
+    oracle parity on the smallest targets."""
     from symplectic_stiefel_nearness_amd import acceptance as dacc
     shapes = inputs.uneven_shapes(128)
     T = [t.cuda() for t in inputs.make_uneven(128)]
-    outs, stats = ssn.solve_uneven(T, return_stats=True)
+    table = ssn.autotune(shapes)
+    say("  autotuned tiles", sorted(table.items())[:6], "...")
+    outs, stats = ssn.solve_uneven(T, tuning=table, return_stats=True)
     assert int((stats["status"] != 0).sum()) == 0
     worst = {"feasibility": 0.0, "stationarity": 0.0, "delta": 0.0}
     for t, (X, _, _) in zip(T, outs):
@@ -101,3 +104,20 @@ def test_autotune_returns_valid_tiles(ssn, tmp_path):
     T = [inputs.make_targets(1024, 8, 1, seeds=[7])[0].cuda(), inputs.make_targets(2048, 48, 1, seeds=[8])[0].cuda()]
     outs, stats = ssn.solve_uneven(T, tuning=table, return_stats=True)
     _check_vs_oracle(T, outs, stats)
+
+
+def test_two_plans_equal_one_plan(ssn):
+    """The two-plan solve (largest order on the current stream, the rest on a side stream, both
+    eager and captured) gives the single-plan solve's results bit for bit."""
+    shapes = [(300, 8), (1024, 48), (200, 64), (2048, 16), (512, 64), (100, 3)]
+    T = [inputs.make_targets(n, k, 1, seeds=[700 + j])[0].cuda() for j, (n, k) in enumerate(shapes)]
+    one = ssn.UnevenSolver(shapes, inputs=T, groups=1)
+    ref = [(X.clone(), om.clone(), it.clone()) for X, om, it in one.run()]
+    two = ssn.UnevenSolver(shapes, inputs=T)
+    assert len(two.groups) == 2 and sorted(two.groups[0]) == [2, 4]
+    cap = ssn.UnevenSolver(shapes, inputs=T, capture=True)
+    for solver in (two, cap, cap):
+        out = solver.run()
+        torch.cuda.synchronize()
+        for (X, om, it), (Xr, omr, itr) in zip(out, ref):
+            assert torch.equal(X, Xr) and torch.equal(om, omr) and torch.equal(it, itr)
