@@ -19,7 +19,7 @@ def test_schedule_counts_and_batch_order():
     warm = [r for r in rows if r[0] == "warmup"]
     meas = [r for r in rows if r[0] == "measured"]
     assert len(warm) == p["single_warmups"] * len(p["single_methods"]) + len(p["batch_methods"])
-    assert [r[3] for r in meas if r[1] == "single"] == [0, 1, 2]
+    assert [r[3] for r in meas if r[1] == "single"] == [pos for pos in range(3) for _ in p["single_methods"]]
     batch1 = [r[2] for r in meas if r[1] == "batch"]
     batch2 = [r[2] for r in experiment.schedule(p, 6, 2) if r[0] == "measured" and r[1] == "batch"]
     assert sorted(batch1) == sorted(p["batch_methods"]) and batch2 == batch1[::-1]
@@ -44,14 +44,16 @@ def _write_rows(tmp_path, p, ms):
 
 def test_summary_statistics(tmp_path):
     p = small_protocol()
-    ms = {("single", "dual"): 1.0, ("batch", "dual"): 2.0, ("batch", "dual_sequential"): 8.0,
-          ("batch", "dual_device"): 1.5}
+    ms = {("single", "dual"): 1.0, ("single", "gao_nonmonotone"): 30.0, ("single", "jensen_rtr2"): 10.0,
+          ("batch", "dual"): 2.0, ("batch", "dual_sequential"): 8.0, ("batch", "dual_device"): 1.5}
     _write_rows(tmp_path, p, ms)
     s = experiment.summarize(tmp_path, p)
     c = s["cases"][0]
     assert c["single"]["dual"]["calls"] == 6 and c["single"]["dual"]["round_mean_ms"] == [1.0, 2.0]
     assert c["batch"]["dual"]["mean_ms"] == pytest.approx(3.0)
     assert c["batch"]["speedup_batched_vs_sequential"] == pytest.approx(4.0)
+    assert c["single"]["gao_nonmonotone"]["speedup_dual_vs_this"] == pytest.approx(30.0)
+    assert c["single"]["jensen_rtr2"]["speedup_dual_vs_this"] == pytest.approx(10.0)
     assert s["all_calls_successful"] and s["accuracy"]["max_delta"] == 3e-12
     assert (tmp_path / "summary.md").read_text().count("| n256_k8 |") == 1
     json.loads((tmp_path / "summary.json").read_text())
@@ -59,8 +61,8 @@ def test_summary_statistics(tmp_path):
 
 def test_summary_rejects_missing_calls(tmp_path):
     p = small_protocol()
-    ms = {("single", "dual"): 1.0, ("batch", "dual"): 2.0, ("batch", "dual_sequential"): 8.0,
-          ("batch", "dual_device"): 1.5}
+    ms = {("single", "dual"): 1.0, ("single", "gao_nonmonotone"): 30.0, ("single", "jensen_rtr2"): 10.0,
+          ("batch", "dual"): 2.0, ("batch", "dual_sequential"): 8.0, ("batch", "dual_device"): 1.5}
     _write_rows(tmp_path, p, ms)
     path = tmp_path / "round2" / "n256_k8" / "raw.jsonl"
     lines = path.read_text().splitlines()
