@@ -198,6 +198,24 @@ def main():
     log("acceptance fp64", acc, "split", acc_split, f"split vs fp64 X: {split_vs_fp64:.2e}")
     del Xs
 
+    # split with the FP64 residual-and-refinement step: every target's acceptance residuals in
+    # FP64 on the device, and an FP64 re-solve of any target outside the thresholds
+    for _ in range(args.warmup):
+        ssn.solve_device(T, split=True, refine=True, check=False)
+    torch.cuda.synchronize()
+    t_r = time.perf_counter()
+    for _ in range(args.steps):
+        out_r = ssn.solve_device(T, split=True, refine=True, check=False, return_stats=True)
+    torch.cuda.synchronize()
+    ms_refine = (time.perf_counter() - t_r) * 1e3 / args.steps
+    res_r = out_r[3]["residuals"]
+    split_refine = {"ms_per_step": ms_refine, "solves_per_s": BATCH / (ms_refine * 1e-3),
+                    "refined_targets": int(out_r[3]["refined"].numel()),
+                    "max_rF": float(res_r["feasibility"].max()), "max_rG": float(res_r["stationarity"].max()),
+                    "max_gap": float(res_r["delta"].max())}
+    log(f"split + FP64 residual-and-refinement: {ms_refine:.3f} ms/step, refined {split_refine['refined_targets']}")
+    del out_r
+
     # the same solve through the single C call
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -394,6 +412,7 @@ def main():
         "split_stage_ms": stages_split,
         "split_acceptance": acc_split,
         "split_vs_fp64_rel_X": split_vs_fp64,
+        "split_refine": split_refine,
         "fwd_bwd_ms_split_forward": fb[True],
         "plain_solves_per_s": BATCH / (plain_ms * 1e-3) if plain_ms else None,
         "single_call_solves_per_s": BATCH / (fused_ms * 1e-3),

@@ -200,7 +200,7 @@ def project(T, split=False):
 
 
 @torch.no_grad()
-def solve_device(T, return_stats=False, check=True, split=False, **caps):
+def solve_device(T, return_stats=False, check=True, split=False, refine=False, **caps):
     """Device-resident solve: T is a float64 HIP tensor [batch, 2n, 2k]; outputs stay on the device.
 
     Returns (X, omega, iterations[, stats]) where stats = dict(status, counters[batch, 12]
@@ -208,11 +208,35 @@ def solve_device(T, return_stats=False, check=True, split=False, **caps):
     eigensolver, setup, CG, line search, whole kernel, 0).
     split=True computes the Gram and the recovery as split BF16 products with FP32
     accumulation reassembled to FP64 (every other step in FP64).
+    refine=True (with split) ends with the FP64 residual-and-refinement step: the acceptance
+    residuals r_F, r_G, delta of every target in FP64 on the device (acceptance.inspect), and an
+    FP64 solve (FP64 Gram and recovery) of the targets outside the thresholds, whose results
+    replace the split ones; stats then also holds `residuals` (the split solve's metrics) and
+    `refined` (indices of the re-solved targets). A dict for `refine` replaces some of the
+    thresholds (keys feasibility, stationarity, delta).
     caps: max_newton, max_cg, max_backtrack, feas_tol; each defaults to the module constant
     of that name (MAX_NEWTON, ...) at call time.
     """
     assert T.is_cuda, "solve_device expects a device tensor"
     X, omega, iters, _, stats = _run_solve(T, split, caps)
+    if split and refine and T.shape[0]:
+        from . import acceptance
+        met = acceptance.inspect(T, X)
+        passed = met["passed"]
+        if isinstance(refine, dict):              # other thresholds than the acceptance ones
+            limits = dict(acceptance.THRESHOLDS, **refine)
+            passed = torch.isfinite(X).flatten(1).all(1)
+            for key, lim in limits.items():
+                passed &= torch.isfinite(met[key]) & (met[key].abs() <= lim)
+        bad = (~passed).nonzero().flatten()
+        if bad.numel():
+            Tb = T[bad]
+            Xb, ob, ib, _, sb = _run_solve(Tb, False, caps)
+            X[bad], omega[bad], iters[bad] = Xb, ob, ib
+            stats["status"][bad] = sb["status"]
+            stats["counters"][bad] = sb["counters"]
+        stats["residuals"] = dict({k: met[k] for k in ("feasibility", "stationarity", "delta")}, passed=passed)
+        stats["refined"] = bad
     if check:
         _raise_for_status(stats["status"])
     if return_stats:

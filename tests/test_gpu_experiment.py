@@ -15,9 +15,11 @@ def test_protocol_small_run(ssn, tmp_path):
     print("experiment", p["cases"], flush=True)
     s = experiment.run(tmp_path / "out", p, log=lambda line: None)
     assert s["all_calls_successful"], s["failure_records"]
-    assert s["counts"]["formal_calls"] == 2 * 2 * (4 + 3)
+    assert s["counts"]["formal_calls"] == 2 * 2 * (4 * len(p["single_methods"]) + len(p["batch_methods"]))
     for c in s["cases"]:
         assert c["batch"]["speedup_batched_vs_sequential"] > 0
+        for method in ("gao_nonmonotone", "jensen_rtr2"):      # the comparison solvers, single target
+            assert c["single"][method]["all_calls_successful"] and c["single"][method]["speedup_dual_vs_this"] > 0
     env = json.loads((tmp_path / "out" / "environment.json").read_text())
     assert env["compute_units"] > 0 and env["library"]
     raw = (tmp_path / "out" / "round1" / "n256_k8" / "raw.jsonl").read_text().splitlines()

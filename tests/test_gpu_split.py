@@ -135,3 +135,22 @@ def test_split_forward_backward(ssn):
     e = rel(Td.grad.cpu().numpy(), ref)
     say(f"  dL/dT after split forward: rel {e:.2e}")
     assert e < 1e-7
+
+
+def test_split_refinement_step(ssn):
+    """The FP64 residual-and-refinement step of the split solve: with the acceptance thresholds
+    no metric target needs a re-solve (results bitwise those of the split solve); with a
+    feasibility threshold no split result can meet, every target is re-solved in FP64 and the
+    results are bitwise those of the FP64 solve."""
+    T = inputs.make_case("n4096_k48", 8).cuda()
+    Xs, os_, its = ssn.solve_device(T, split=True)
+    X, om, it, st = ssn.solve_device(T, split=True, refine=True, return_stats=True)
+    say("  residuals: max rF %.2e rG %.2e delta %.2e, refined %s" % (
+        float(st["residuals"]["feasibility"].max()), float(st["residuals"]["stationarity"].max()),
+        float(st["residuals"]["delta"].max()), st["refined"].tolist()))
+    assert st["refined"].numel() == 0 and bool(st["residuals"]["passed"].all())
+    assert torch.equal(X, Xs) and torch.equal(om, os_) and torch.equal(it, its)
+    Xf, of, itf = ssn.solve_device(T)
+    X2, o2, it2, st2 = ssn.solve_device(T, split=True, refine=dict(feasibility=1e-300), return_stats=True)
+    assert st2["refined"].tolist() == list(range(8))
+    assert torch.equal(X2, Xf) and torch.equal(o2, of) and torch.equal(it2, itf)
