@@ -64,13 +64,20 @@ def cpu_baseline(budget_s=12.0):
 PROFILE = ROOT / "profiles" / "solve_fp64_n4096_k48_b64_kernel_stats.md"
 
 
-def profiled_traffic(kernel, grid_size):
-    """Copy-calibrated HBM bytes per launch of `kernel` at `grid_size` threads, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (None if it is not there)."""
+def profiled_traffic(kernel, grid):
+    """Copy-calibrated HBM bytes per launch of `kernel` at launch shape `grid` ("x x y" threads, as
+    the kernel trace records it), from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    this workload (None if it is not there)."""
     try:
+        section = None
         for line in PROFILE.read_text(encoding="utf-8").splitlines():
+            if line.startswith("## "):
+                section = line
+                continue
+            if section is None or not section.startswith("## HBM traffic"):
+                continue
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) >= 5 and cells[0] == f"`{kernel}`" and cells[1] == str(grid_size):
+            if len(cells) >= 5 and cells[0] == f"`{kernel}`" and cells[1] == grid:
                 return float(cells[4]) * 1e6
     except (OSError, ValueError):
         pass
@@ -376,10 +383,10 @@ def main():
                      roof_hbm("ssn::split_recover_kernel<6,5> (+ split_recovery_fragments)", b_rec, f_rec,
                               stages_split["recover"])]
     # HBM traffic per launch of each kernel (the main kernel of a stage) at this workload's grid
-    for k, (pname, grid) in zip(kernels, (("void ssn::gram_partial_kernel<6>", 4096 * BATCH),
-                                          ("void ssn::eigh_kernel<3, false, true>", 512 * BATCH),
-                                          ("void ssn::dual_kernel<6>", 512 * BATCH),
-                                          ("void ssn::recover_lds_kernel<6>", 2048 * BATCH))):
+    for k, (pname, grid) in zip(kernels, (("void ssn::gram_partial_kernel<6>", f"4096 x {BATCH}"),
+                                          ("void ssn::eigh_kernel<3, false, true>", f"{512 * BATCH} x 1"),
+                                          ("void ssn::dual_kernel<6>", f"{512 * BATCH} x 1"),
+                                          ("void ssn::recover_lds_kernel<6>", f"2048 x {BATCH}"))):
         k["traffic"] = profiled_traffic(pname, grid)
         if k["traffic"] is not None:
             k["traffic_source"] = (f"{PROFILE.relative_to(ROOT)}: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "

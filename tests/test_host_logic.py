@@ -13,3 +13,22 @@ def test_adjoint_status_check_raises():
     bad[1, 1] = 1
     with pytest.raises(RuntimeError, match=r"targets \[1\]"):
         dual._raise_for_adjoint(bad)
+
+
+def test_bench_reads_traffic_of_the_timed_launch_shapes():
+    """bench.py takes each stage kernel's HBM bytes per launch from the committed rocprofv3
+    summary, by kernel and launch shape: every stage kernel of the 64-target step has a row, the
+    value comes from the traffic table (not the trace table of the same shapes), and the
+    16-target launch of the same kernel is a different row."""
+    import bench
+    shapes = (("void ssn::gram_partial_kernel<6>", "4096 x 64"),
+              ("void ssn::eigh_kernel<3, false, true>", "32768 x 1"),
+              ("void ssn::dual_kernel<6>", "32768 x 1"),
+              ("void ssn::recover_lds_kernel<6>", "2048 x 64"))
+    got = {k: bench.profiled_traffic(k, g) for k, g in shapes}
+    assert all(v is not None and v > 0 for v in got.values()), got
+    # the recovery reads T (403 MB at 64 targets) and writes X (403 MB): at least that much moves
+    assert got["void ssn::recover_lds_kernel<6>"] >= 0.95 * 2 * 64 * 8192 * 96 * 8
+    assert bench.profiled_traffic("void ssn::recover_lds_kernel<6>", "8192 x 16") < \
+        0.5 * got["void ssn::recover_lds_kernel<6>"]
+    assert bench.profiled_traffic("void ssn::no_such_kernel", "1 x 1") is None
