@@ -12,8 +12,8 @@ comparison solvers of the reference experiment (Riemannian steepest descent and 
 `baselines`) are PyTorch code, off the hot path.
 """
 from .dual import (initialize_kernels, solve, solve_device, solve_batches, project, backward_device, gram_chunks,
-                   dual_batch, dual_batch_staged, recover_matmul, FEAS_TOL, MAX_NEWTON, MAX_CG,
-                   MAX_BACKTRACK)
+                   dual_batch, dual_batch_staged, recover_matmul, residuals, FEAS_TOL, MAX_NEWTON, MAX_CG,
+                   MAX_BACKTRACK, REFINE_THRESHOLDS)
 
 from .uneven import UnevenSolver, solve_uneven
 from .autotune import autotune
@@ -21,5 +21,5 @@ from .baselines import (SolverOptions, TrustRegionOptions, SolveResult, solve_ga
                         solve_jensen_trustregions)
 
 __all__ = ["initialize_kernels", "solve", "solve_device", "solve_batches", "project", "backward_device", "gram_chunks",
-           "dual_batch", "dual_batch_staged", "recover_matmul", "UnevenSolver", "solve_uneven", "autotune",
+           "dual_batch", "dual_batch_staged", "recover_matmul", "residuals", "UnevenSolver", "solve_uneven", "autotune",
            "SolverOptions", "TrustRegionOptions", "SolveResult", "solve_gao_nonmonotone", "solve_jensen_trustregions"]

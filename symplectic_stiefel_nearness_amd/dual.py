@@ -19,6 +19,9 @@ FEAS_TOL = 1e-8                        # dual_recommended_hvp.py:15
 MAX_NEWTON, MAX_CG, MAX_BACKTRACK = 100, 150, 40   # dual_recommended_hvp.py:16
 PINNED_OUTPUT_MIN_BYTES = 64 * 2 ** 20  # dual_recommended_hvp.py:18
 MAX_ORDER = 128                        # largest 2k handled by this build
+# r_F and r_G limits of the split solve's refinement step: the acceptance thresholds
+# (reproduce.py:28)
+REFINE_THRESHOLDS = {"feasibility": 1e-8, "stationarity": 1e-9}
 
 MSG_NO_DESCENT = "批量线搜索未找到下降步"      # dual_recommended_hvp.py:190 (line search found no descent step)
 MSG_NEWTON_CAP = "达到批量 Newton 迭代上限"     # dual_recommended_hvp.py:191 (Newton iteration cap reached)
@@ -208,26 +211,26 @@ def solve_device(T, return_stats=False, check=True, split=False, refine=False, *
     eigensolver, setup, CG, line search, whole kernel, 0).
     split=True computes the Gram and the recovery as split BF16 products with FP32
     accumulation reassembled to FP64 (every other step in FP64).
-    refine=True (with split) ends with the FP64 residual-and-refinement step: the acceptance
-    residuals r_F, r_G, delta of every target in FP64 on the device (acceptance.inspect), and an
-    FP64 solve (FP64 Gram and recovery) of the targets outside the thresholds, whose results
-    replace the split ones; stats then also holds `residuals` (the split solve's metrics) and
+    refine=True (with split) ends with the FP64 residual-and-refinement step: the residuals of
+    every target in FP64 on the device (`residuals`: r_F, and the KKT residual that bounds r_G
+    from above), and an FP64 solve (FP64 Gram and recovery) of the targets outside
+    REFINE_THRESHOLDS (the acceptance thresholds of r_F and r_G), whose results replace the
+    split ones; stats then also holds `residuals` (the split solve's values and `passed`) and
     `refined` (indices of the re-solved targets). A dict for `refine` replaces some of the
-    thresholds (keys feasibility, stationarity, delta).
+    thresholds (keys feasibility, stationarity).
     caps: max_newton, max_cg, max_backtrack, feas_tol; each defaults to the module constant
     of that name (MAX_NEWTON, ...) at call time.
     """
     assert T.is_cuda, "solve_device expects a device tensor"
+    if isinstance(refine, dict) and set(refine) - set(REFINE_THRESHOLDS):
+        raise ValueError(f"refine thresholds: unknown keys {sorted(set(refine) - set(REFINE_THRESHOLDS))}")
     X, omega, iters, _, stats = _run_solve(T, split, caps)
     if split and refine and T.shape[0]:
-        from . import acceptance
-        met = acceptance.inspect(T, X)
-        passed = met["passed"]
-        if isinstance(refine, dict):              # other thresholds than the acceptance ones
-            limits = dict(acceptance.THRESHOLDS, **refine)
-            passed = torch.isfinite(X).flatten(1).all(1)
-            for key, lim in limits.items():
-                passed &= torch.isfinite(met[key]) & (met[key].abs() <= lim)
+        res = residuals(T, X, omega)
+        limits = dict(REFINE_THRESHOLDS, **refine) if isinstance(refine, dict) else REFINE_THRESHOLDS
+        passed = torch.isfinite(X).flatten(1).all(1)
+        for key, lim in limits.items():
+            passed &= torch.isfinite(res[key]) & (res[key] <= lim)
         bad = (~passed).nonzero().flatten()
         if bad.numel():
             Tb = T[bad]
@@ -235,7 +238,7 @@ def solve_device(T, return_stats=False, check=True, split=False, refine=False, *
             X[bad], omega[bad], iters[bad] = Xb, ob, ib
             stats["status"][bad] = sb["status"]
             stats["counters"][bad] = sb["counters"]
-        stats["residuals"] = dict({k: met[k] for k in ("feasibility", "stationarity", "delta")}, passed=passed)
+        stats["residuals"] = dict(res, passed=passed)
         stats["refined"] = bad
     if check:
         _raise_for_status(stats["status"])
@@ -541,6 +544,43 @@ def dual_batch_staged(Cr, Ci, events=None, check=True):
     if check:
         _raise_for_status(status)
     return omega, P, Q, iters, dict(status=status, counters=counters)
+
+
+@torch.no_grad()
+def residuals(T, X, omega):
+    """FP64 residuals of solutions X, with multipliers omega, of device targets T, from this
+    library's own Gram and recovery kernels:
+
+        feasibility   r_F = ||X^T J X - J|| / sqrt(m), where X^T J X is the Ci of the Gram of X;
+        stationarity  ||(X - T) - J X omega|| / max(1, ||T||), the KKT residual with the solve's
+                      own multiplier.
+
+    The acceptance check's r_G (experiment.py:105-137) is the same residual with the multiplier
+    that minimises it (the tangent projection), so `stationarity` bounds r_G from above and
+    equals it to rounding at a solution. (X - T) - J X omega = [A - D omega; D + A omega] - T
+    for X = [A; D] is the recovery of X with P = I and Q = -omega^T.
+    Returns dict(feasibility, stationarity) of float64 [batch] device tensors."""
+    lib = _require_ready()
+    batch, n, m = _check_targets(T)
+    assert X.shape == T.shape and X.dtype == torch.float64 and X.device == T.device
+    assert omega.shape == (batch, m, m) and omega.dtype == torch.float64 and omega.device == T.device
+    dev = T.device
+    k = m // 2
+    J = torch.zeros((m, m), dtype=torch.float64, device=dev)
+    J[:k, k:] = torch.eye(k, dtype=torch.float64, device=dev)
+    J[k:, :k] = -torch.eye(k, dtype=torch.float64, device=dev)
+    _, CiX = gram_chunks(X)
+    feasibility = torch.linalg.vector_norm(CiX - J, dim=(-2, -1)) / m ** 0.5
+    mp = lib.ssn_padded_order(m)
+    P = torch.zeros((batch, mp, mp), dtype=torch.float64, device=dev)
+    P[:, :m, :m] = torch.eye(m, dtype=torch.float64, device=dev)
+    Q = torch.zeros_like(P)
+    Q[:, :m, :m] = -omega.mT
+    R = recover_matmul(X, P, Q)
+    R.sub_(T)
+    scale = torch.linalg.vector_norm(T, dim=(-2, -1)).clamp_min(1.0)
+    stationarity = torch.linalg.vector_norm(R, dim=(-2, -1)) / scale
+    return dict(feasibility=feasibility, stationarity=stationarity)
 
 
 @torch.no_grad()
