@@ -377,7 +377,7 @@ def main():
                 "fp64_equivalent_tflops": flops / (ms * 1e-3) / 1e12, "avg_ms": ms}
 
     kernels = [roof_mfma("ssn::gram_partial_kernel<6> (+ gram_reduce_tiles_kernel)", f_gram, gram_ms),
-               roof_mfma("ssn::eigh_kernel<3,false,true> (+ eigh_replay_kernel)", f_eigh, eigh_ms),
+               roof_mfma("ssn::eigh_coop_kernel<3> (rotations and eigenvector replay in one launch)", f_eigh, eigh_ms),
                roof_mfma("ssn::dual_kernel<6>", fd, newton_ms),
                roof_mfma("ssn::recover_lds_kernel<6>", f_rec, rec_ms)]
     kernels_split = [roof_hbm("ssn::split_gram_kernel<6,2,5> (+ gram_reduce_tiles_kernel)", b_gram, f_gram,
@@ -386,7 +386,7 @@ def main():
                               stages_split["recover"])]
     # HBM traffic per launch of each kernel (the main kernel of a stage) at this workload's grid
     for k, (pname, grid) in zip(kernels, (("void ssn::gram_partial_kernel<6>", f"4096 x {BATCH}"),
-                                          ("void ssn::eigh_kernel<3, false, true>", f"{512 * BATCH} x 1"),
+                                          ("void ssn::eigh_coop_kernel<3>", f"{512 * BATCH * 4} x 1"),
                                           ("void ssn::dual_kernel<6>", f"{512 * BATCH} x 1"),
                                           ("void ssn::recover_lds_kernel<6>", f"2048 x {BATCH}"))):
         k["traffic"] = profiled_traffic(pname, grid)
@@ -394,11 +394,11 @@ def main():
             k["traffic_source"] = (f"{PROFILE.relative_to(ROOT)}: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
                                    f"(factors from a 256 MiB device copy) of {pname}, bytes per launch")
     dominant = max(kernels, key=lambda k: k["avg_ms"])
-    # the dual engine and the eigensolver run one workgroup (one CU) per target: also state the
-    # fraction of the FP64 MFMA peak of the CUs the launch occupies
+    # the dual engine runs one workgroup (one CU) per target: also state the fraction of the FP64
+    # MFMA peak of the CUs the launch occupies (the one-launch eigensolver occupies every CU)
     n_cu = torch.cuda.get_device_properties(0).multi_processor_count
     for k in kernels:
-        if "dual_kernel" in k["kernel"] or "eigh_kernel" in k["kernel"]:
+        if "dual_kernel" in k["kernel"]:
             k["occupied_cus"] = min(BATCH, n_cu)
             k["frac_of_occupied_cus"] = k["achieved"] / (FP64_PEAK_TFLOPS * min(BATCH, n_cu) / n_cu)
 

@@ -119,6 +119,10 @@ typedef struct ssn_options {
     int max_backtrack;   /* Armijo trials per Newton step, 0 .. 1100; exhausting them sets SSN_ST_NO_DESCENT */
     int split;           /* 0: FP64 Gram and recovery; 1: split precision (ssn_solve_split)                */
     double feas_tol;     /* a target stops once its scaled residual < feas_tol / 2; finite, >= 0           */
+    int shared_device;   /* 0: the eigensolver may take every CU for a moment (its one-launch form, which
+                            overlaps the eigenvector replay with the rotations); 1: other work runs on
+                            the device beside this solve (several batches in flight): the two-launch
+                            form, which leaves CUs free. Same results bit for bit either way.          */
 } ssn_options;
 
 void ssn_default_options(ssn_options* opts);

@@ -44,7 +44,7 @@ class SsnOptions(ctypes.Structure):
     """Mirror of `ssn_options` (include/ssn.h)."""
     _fields_ = [("max_newton", ctypes.c_int), ("max_cg", ctypes.c_int),
                 ("max_backtrack", ctypes.c_int), ("split", ctypes.c_int),
-                ("feas_tol", ctypes.c_double)]
+                ("feas_tol", ctypes.c_double), ("shared_device", ctypes.c_int)]
 
 
 _OP = ctypes.POINTER(SsnOptions)

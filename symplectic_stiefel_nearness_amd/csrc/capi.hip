@@ -370,7 +370,11 @@ int ssn_solve_static(void* stream, const double* T, long long sT_b, long long sT
     void* ws = nullptr;
     rc = static_scratch(dev, &ws);
     if (rc) return rc;
-    return solve_impl(false, Caps(), stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo, iters,
+    // the eigensolver's two-launch form: plain kernel launches only (its one-launch form is a
+    // cooperative launch, which graph capture and allocation-free callers are not given)
+    Caps caps;
+    caps.eigh_whole_device = 0;
+    return solve_impl(false, caps, stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo, iters,
                       status, counters, ws, SSN_STATIC_SCRATCH_BYTES);
 }
 
@@ -383,6 +387,7 @@ void ssn_default_options(ssn_options* opts)
     opts->max_backtrack = d.max_backtrack;
     opts->split = 0;
     opts->feas_tol = d.feas_tol;
+    opts->shared_device = 0;
 }
 
 int ssn_solve_opts(void* stream, const ssn_options* opts, const double* T, long long sT_b, long long sT_r,
@@ -394,13 +399,14 @@ int ssn_solve_opts(void* stream, const ssn_options* opts, const double* T, long 
     constexpr int CAP_LIMIT = 1 << 20;
     if (opts->max_newton < 0 || opts->max_newton > CAP_LIMIT || opts->max_cg < 0 || opts->max_cg > CAP_LIMIT ||
         opts->max_backtrack < 0 || opts->max_backtrack > 1100 || (opts->split != 0 && opts->split != 1) ||
-        !(opts->feas_tol >= 0.0) || !(opts->feas_tol < 1e300))
+        !(opts->feas_tol >= 0.0) || !(opts->feas_tol < 1e300) || (opts->shared_device != 0 && opts->shared_device != 1))
         return SSN_ERR_ARGS;
     Caps caps;
     caps.max_newton = opts->max_newton;
     caps.max_cg = opts->max_cg;
     caps.max_backtrack = opts->max_backtrack;
     caps.feas_tol = opts->feas_tol;
+    caps.eigh_whole_device = opts->shared_device ? 0 : 1;
     return solve_impl(opts->split != 0, caps, stream, T, sT_b, sT_r, batch, n, m, X, sX_b, sX_r, omega, sO_b, ldo,
                       iters, status, counters, ws, ws_bytes);
 }

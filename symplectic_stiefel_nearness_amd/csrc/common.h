@@ -74,6 +74,9 @@ struct Caps {
     int max_cg = 150;           // CG iterations per Newton step
     int max_backtrack = 40;     // Armijo trials per Newton step; exhausting them is ST_NO_DESCENT
     double feas_tol = 1e-8;     // a target stops when its scaled residual < feas_tol / 2
+    // launch form, not a cap: the eigensolver may take every CU for its one-launch form (1), or
+    // must leave CUs to other work running beside the solve (0: the two-launch form)
+    int eigh_whole_device = 1;
 };
 
 // Status codes shared by kernels and the C ABI (include/ssn.h mirrors them).

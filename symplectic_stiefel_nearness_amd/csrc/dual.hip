@@ -5,7 +5,7 @@
 namespace ssn {
 
 hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int ldc, int batch,
-                       int m, double* ws, size_t lds_limit);
+                       int m, double* ws, size_t lds_limit, bool whole_device);
 
 template <int NT>
 __global__ __launch_bounds__(DT)
@@ -61,7 +61,7 @@ hipError_t launch_dual(hipStream_t stream, const double* Cr, const double* Ci, l
                        double* Pf, double* Qf, long long* iters, int* status, int* counters,
                        double* ws, size_t lds_limit, const Caps& caps)
 {
-    hipError_t err = launch_eigh(stream, Cr, sC_b, ldc, batch, m, ws, lds_limit);
+    hipError_t err = launch_eigh(stream, Cr, sC_b, ldc, batch, m, ws, lds_limit, caps.eigh_whole_device != 0);
     if (err != hipSuccess) return err;
     return launch_dual_newton(stream, Cr, Ci, sC_b, ldc, batch, m, omega, sO_b, ldo, Pf, Qf,
                               iters, status, counters, ws, lds_limit, caps);

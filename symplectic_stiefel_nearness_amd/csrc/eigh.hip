@@ -3,6 +3,7 @@
 // steps for the uneven-batch plan. Replaces torch.linalg.eigh at
 // src/symplectic_projection/dual_recommended_hvp.py:108.
 #include "engine.h"
+#include <cstdlib>
 
 namespace ssn {
 
@@ -21,8 +22,98 @@ static size_t eigh_lds_doubles(int m, bool v_in_lds) {
 #define SSN_EIGH_REPLAY 1
 #endif
 
+// ---------------------------------------------------------------- eigensolver and replay in one launch
+// The eigensolver workgroups (blockIdx.x < batch) publish the rotation record as they go, and
+// the replay workgroups (RF_ROWS rows of one target each) follow it, so that V is ready a chunk
+// of rounds after the last rotation instead of one replay launch later. Every workgroup takes
+// more than half of a CU's LDS, so each has a CU to itself, and the launch is cooperative, so
+// all of them are resident at once: every wait of a replay workgroup is on an eigensolver
+// workgroup that is running (and that never waits on the replay).
+// The replay's staging buffer lives in the dynamic LDS (which the replay workgroups do not
+// otherwise use), not in static LDS: static LDS would sit below the eigensolver's area and
+// move its base, and the eigensolver measured 3x slower with its area 32 KB higher.
+template <int JB>
+__global__ __launch_bounds__(ET)
+void eigh_coop_kernel(EighArgs a, int batch, int rpt, long long spin_limit)
+{
+    if ((int)blockIdx.x < batch) {
+        eigh_body<JB, false, true, true>(a, blockIdx.x);
+    } else {
+        extern __shared__ double dyn_lds[];
+        double2 (*par)[RP_CHUNK * 64] = reinterpret_cast<double2 (*)[RP_CHUNK * 64]>(dyn_lds);
+        unsigned long long* shared_p = reinterpret_cast<unsigned long long*>(dyn_lds + 2 * 2 * RP_CHUNK * 64);
+        const int w = (int)blockIdx.x - batch, b = w / rpt, rb = w - b * rpt;
+        eigh_replay_follow(a.ws + (long long)b * target_doubles(a.mp), a.m, a.mp, rb, spin_limit, par, shared_p);
+    }
+}
+
+// each target's progress word and replay-failure flag, cleared before the cooperative launch
+__global__ __launch_bounds__(256)
+void eigh_progress_clear_kernel(double* ws, int mp, int batch)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= batch) return;
+    double* extra = ws + (long long)b * target_doubles(mp) + (long long)S_COUNT * mp * mp + mp;
+    reinterpret_cast<unsigned long long*>(extra)[EX_PROGRESS] = 0ull;
+    extra[EX_REPLAY_FAILED] = 0.0;
+}
+
+// over half of a CU's LDS (one workgroup per CU), and room for the replay's staging buffer
+constexpr size_t COOP_MIN_DYN_LDS = 96 * 1024;
+
+// Polls a replay workgroup makes before it gives up on its eigensolver (seconds of waiting;
+// SSN_REPLAY_SPIN_LIMIT overrides it, e.g. 0 to exercise the failure path).
+static long long replay_spin_limit() {
+    const char* e = getenv("SSN_REPLAY_SPIN_LIMIT");
+    return e ? atoll(e) : (1LL << 22);
+}
+
+// CUs and cooperative-launch support of the current device, looked up once per device
+static bool coop_grid_fits(int wgs) {
+    constexpr int MAXDEV = 64;
+    static int cus[MAXDEV] = {}, coop[MAXDEV] = {}, known[MAXDEV] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAXDEV) return false;
+    if (!known[dev]) {
+        int c = 0, k = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) c = 0;
+        if (hipDeviceGetAttribute(&k, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess) k = 0;
+        cus[dev] = c; coop[dev] = k; known[dev] = 1;
+    }
+    return coop[dev] && wgs <= cus[dev];
+}
+
+// The one-launch form when it fits (one workgroup per CU for every eigensolver and replay
+// workgroup); returns hipErrorCooperativeLaunchTooLarge when it does not, so that the caller
+// takes the two-launch form. SSN_EIGH_COOP=0 disables it.
+template <int JB>
+static hipError_t launch_eigh_coop(hipStream_t stream, const EighArgs& e, int batch, size_t lds, size_t lds_limit) {
+    const char* env = getenv("SSN_EIGH_COOP");
+    if (env && atoi(env) == 0) return hipErrorCooperativeLaunchTooLarge;
+    const int rpt = (e.mp + RF_ROWS - 1) / RF_ROWS;
+    const int grid = batch * (1 + rpt);
+    const size_t dyn = lds > COOP_MIN_DYN_LDS ? lds : COOP_MIN_DYN_LDS;
+    static_assert(sizeof(double2) * 2 * RP_CHUNK * 64 + sizeof(unsigned long long) <= COOP_MIN_DYN_LDS, "staging");
+    if (dyn > lds_limit || !coop_grid_fits(grid)) return hipErrorCooperativeLaunchTooLarge;
+    const void* fn = (const void*)eigh_coop_kernel<JB>;
+    hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(eigh_progress_clear_kernel, dim3((batch + 255) / 256), dim3(256), 0, stream, e.ws, e.mp, batch);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    EighArgs a = e;
+    int nb = batch, r = rpt;
+    long long spin = replay_spin_limit();
+    void* args[] = {&a, &nb, &r, &spin};
+    err = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(ET), args, (unsigned)dyn, stream);
+    if (err != hipSuccess) (void)hipGetLastError();      // the two-launch form follows
+    return err;
+}
+
+// whole_device: the one-launch form (eigensolver and replay together, every CU for the length
+// of the eigensolver) may be used; otherwise, or when it does not fit, the two launches.
 hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int ldc, int batch,
-                              int m, double* ws, size_t lds_limit) {
+                              int m, double* ws, size_t lds_limit, bool whole_device) {
     if (SSN_EIGH_REPLAY) {
         // rotations only in the per-target kernel, then the eigenvectors replayed row by row
         const size_t lds = eigh_lds_doubles(m, false) * 8;
@@ -33,6 +124,12 @@ hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int
         const int blocks = (np * (np - 1) / 2 + JU - 1) / JU;
         if (blocks > JB_MAX) return LAUNCH_UNSUPPORTED;
         const bool small = blocks <= JB_BATCH;
+        if (batch > 0 && whole_device) {
+            const hipError_t ce = small ? launch_eigh_coop<JB_BATCH>(stream, e, batch, lds, lds_limit)
+                                        : launch_eigh_coop<JB_MAX>(stream, e, batch, lds, lds_limit);
+            if (ce == hipSuccess) return hipSuccess;
+            if (ce != hipErrorCooperativeLaunchTooLarge) return ce;
+        }
         const void* fn = small ? (const void*)eigh_kernel<JB_BATCH, false, true> : (const void*)eigh_kernel<JB_MAX, false, true>;
         hipError_t err = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (err != hipSuccess) return err;
@@ -68,7 +165,7 @@ hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int
 hipError_t launch_dual_eigh(hipStream_t stream, const double* Cr, long long sC_b, int ldc, int batch, int m,
                             double* ws, size_t lds_limit)
 {
-    return launch_eigh(stream, Cr, sC_b, ldc, batch, m, ws, lds_limit);
+    return launch_eigh(stream, Cr, sC_b, ldc, batch, m, ws, lds_limit, true);
 }
 
 // ---------------------------------------------------------------- uneven batches

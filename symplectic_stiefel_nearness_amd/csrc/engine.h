@@ -372,6 +372,17 @@ __device__ __forceinline__ double rsqrt_refined(double d) {
 //
 // A rotation is applied when |a_pq| > eps sqrt(|a_pp a_qq|); the iteration stops after
 // the first sweep at whose end no pair exceeds that threshold.
+// Eigenvector replay (eigh_replay_body, eigh_replay_follow) and its hand-off from the
+// eigensolver: rows per replay workgroup, rounds staged per LDS chunk, and the progress word
+// (per target, in extra slot EX_PROGRESS: rounds recorded so far, PROGRESS_FINAL once every
+// round and the eigenvalue ranks are in the record).
+constexpr int RP_ROWS = 8;          // rows (waves) per workgroup of eigh_replay_kernel
+constexpr int RP_CHUNK = 16;        // rounds staged per LDS chunk (and published per step)
+constexpr int MAX_ORDER_PADDED = 128; // largest padded order (NT = 8)
+constexpr int EX_PROGRESS = 8;      // extra slot: progress word (unsigned long long)
+constexpr int EX_REPLAY_FAILED = 9; // extra slot: nonzero when a following replay gave up
+constexpr unsigned long long PROGRESS_FINAL = 1ull << 40;
+
 constexpr int JW = 64;        // wave 0: diagonal blocks and rotation parameters
 constexpr int JU = 512 - JW;  // threads that update off-diagonal blocks and V
 constexpr int JB_MAX = 5;     // off-diagonal blocks per update thread: ceil(64*63/2 / 448)
@@ -420,9 +431,14 @@ __device__ __forceinline__ void jac_block(double2 r1, double2 r2, double x00, do
 // (-1 without convergence) and in `fin` the buffer holding the final A.
 // REPLAY: V is not formed here; the update threads of slot pair s (the first group) store the
 // round's (cos, sin) of pair s at rot[(sweep (m - 1) + round) np + s] for eigh_replay_kernel.
-template <int JB, bool V_IN_LDS, bool REPLAY = false>
+// PUBLISH (with REPLAY): the record is written with agent-scope stores at the start of each
+// round, and every RP_CHUNK rounds the first update thread waits for its wave's stores and then
+// stores the number of rounds recorded into *progress, for replay workgroups that run at the
+// same time (eigh_replay_follow).
+template <int JB, bool V_IN_LDS, bool REPLAY = false, bool PUBLISH = false>
 __device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, int* flag, const double** fin,
-                      double2* rot = nullptr) {
+                      double2* rot = nullptr, unsigned long long* progress = nullptr) {
+    static_assert(!PUBLISH || REPLAY, "publishing needs the rotation record");
     const int m = c.m, np = m / 2, nb = np * (np + 1) / 2;
     double* A0 = J;
     double* A1 = J + 4 * nb;
@@ -565,6 +581,17 @@ __device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, in
                         }
                     }
                 }
+                if constexpr (PUBLISH) {
+                    // the record first, so that it has long left when the count is published
+                    if (ut < np) {
+                        const double2 rv = cso[ut];
+                        unsigned long long* d = reinterpret_cast<unsigned long long*>(rot + (sweep * (m - 1) + round) * np + ut);
+                        __hip_atomic_store(d, (unsigned long long)__double_as_longlong(rv.x), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(d + 1, (unsigned long long)__double_as_longlong(rv.y), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
                 // Addresses are formed (and made opaque) while gathering; the stores use
                 // them as plain registers: an asm statement between stores would force a
                 // wait for every outstanding LDS operation.
@@ -593,7 +620,16 @@ __device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, in
                         An[wa[u][0]] = z00; An[wa[u][1]] = z01; An[wa[u][2]] = z10; An[wa[u][3]] = z11;
                     }
                 }
-                if constexpr (REPLAY) {
+                if constexpr (PUBLISH) {
+                    // every record store of this wave (all of them: np <= 64) completes before
+                    // the count that covers it is visible
+                    const int done = sweep * (m - 1) + round + 1;
+                    if (ut == 0 && done % RP_CHUNK == 0) {
+                        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                        __hip_atomic_store(progress, (unsigned long long)done, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                } else if constexpr (REPLAY) {
                     if (ut < np) rot[(sweep * (m - 1) + round) * np + ut] = cso[ut];
                 } else if (!V_IN_LDS && v_on) {
                     const double2 rv = cso[opaque(vS)];
@@ -856,7 +892,10 @@ __host__ __device__ __forceinline__ long long rot_record_offset(int mp) {
 }
 
 // Body of target b's eigensolver workgroup (ET threads, dynamic LDS eigh_lds_doubles).
-template <int JB, bool V_IN_LDS, bool REPLAY = false>
+// PUBLISH: the rotation record, the ranks and the count of recorded rounds go out for replay
+// workgroups running at the same time (eigh_replay_follow); the progress word in extra slot
+// EX_PROGRESS (and EX_REPLAY_FAILED) must have been cleared before the launch.
+template <int JB, bool V_IN_LDS, bool REPLAY = false, bool PUBLISH = false>
 __device__ __forceinline__ void eigh_body(const EighArgs& a, int b)
 {
     extern __shared__ double lds[];
@@ -906,7 +945,8 @@ __device__ __forceinline__ void eigh_body(const EighArgs& a, int b)
     const int vdummy = V_IN_LDS ? m * LV : -(mp * mp);
     double2* rot = reinterpret_cast<double2*>(ws + rot_record_offset(mp));
     int* granks = reinterpret_cast<int*>(ws + rot_record_offset(mp) + (long long)MAX_SWEEPS * (mp - 1) * mp);
-    const int sweeps = jacobi<JB, V_IN_LDS, REPLAY>(c, J, Vj, LV, vdummy, flag, &Af, rot);
+    unsigned long long* progress = reinterpret_cast<unsigned long long*>(extra + EX_PROGRESS);
+    const int sweeps = jacobi<JB, V_IN_LDS, REPLAY, PUBLISH>(c, J, Vj, LV, vdummy, flag, &Af, rot, progress);
     auto eig = [&](int x) {
         const int px = x < np ? x : m - 1 - x;
         const int bi = 2 * (px * np - (px * (px - 1)) / 2);
@@ -920,9 +960,22 @@ __device__ __forceinline__ void eigh_body(const EighArgs& a, int b)
             rank += (lj < li || (lj == li && j < i)) ? 1 : 0;
         }
         pairs[i] = rank;
-        if constexpr (REPLAY) granks[i] = rank;
+        if constexpr (REPLAY && !PUBLISH) granks[i] = rank;
     }
     sync();
+    if constexpr (PUBLISH) {
+        // wave 1 (which wrote the whole record) writes the ranks, waits for all of its stores
+        // and publishes the final count
+        if (c.wave == 1) {
+            for (int i = c.lane; i < m; i += WAVE)
+                __hip_atomic_store(granks + i, pairs[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (c.lane == 0) {
+                const unsigned long long total = (unsigned long long)(sweeps < 0 ? MAX_SWEEPS : sweeps) * (m - 1);
+                __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                __hip_atomic_store(progress, total | PROGRESS_FINAL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
     for (int i = c.tid; i < mp; i += ET) {
         if (i < m) lam[pairs[i]] = eig(i);
         else lam[i] = 1.0;
@@ -939,6 +992,7 @@ __device__ __forceinline__ void eigh_body(const EighArgs& a, int b)
         extra[1] = (double)sweeps;
         extra[2] = (double)(wall_clock64() - t0);
         extra[3] = V_IN_LDS ? 1.0 : 0.0;
+        if constexpr (!PUBLISH) extra[EX_REPLAY_FAILED] = 0.0;   // no following replay to fail
     }
 }
 
@@ -957,9 +1011,6 @@ void eigh_kernel(EighArgs a)
 // the same values, so V is bitwise the in-kernel one) and then moves the tournament (the entry at
 // slot x goes to slot x + 1 over 1 .. m - 1, m - 1 to 1): top entries shift up one lane, bottom
 // entries down one lane. At the end the columns are ordered by the eigenvalue ranks.
-constexpr int RP_ROWS = 8;          // rows (waves) per workgroup
-constexpr int RP_CHUNK = 16;        // rounds staged per LDS chunk
-constexpr int MAX_ORDER_PADDED = 128; // largest padded order (NT = 8)
 // Rows [8 blk, 8 blk + 8) of the target whose solver block is ws (order m, padded mp); the whole
 // workgroup returns at once when blk is past the target's rows.
 __device__ __forceinline__ void eigh_replay_body(const double* ws, int m, int mp, int blk, double2 (*par)[RP_CHUNK * 64])
@@ -1028,6 +1079,114 @@ __device__ __forceinline__ void eigh_replay_body(const double* ws, int m, int mp
     }
 }
 
+
+// Rows [32 rb, 32 rb + 32) of target V's replay while the target's eigensolver workgroup is
+// still rotating (the workgroups of one cooperative launch, eigh_coop_kernel). Each wave holds
+// four rows (the lanes as in eigh_replay_body, with the same expressions per row, so V is
+// bitwise the other replay's). Before staging a chunk of RP_CHUNK rounds, thread 0 polls the
+// target's progress word (agent-scope loads, with a sleep between polls) until the chunk is
+// recorded or the final count is out; the record and the ranks are read with agent-scope
+// loads. A wait that exceeds `spin_limit` polls marks the target failed (extra slot
+// EX_REPLAY_FAILED, which the dual engine turns into ST_EIGH) and ends the workgroup, so
+// every wait ends. Returns false on that failure.
+constexpr int RF_ROWS_PER_WAVE = 4;
+constexpr int RF_ROWS = RF_ROWS_PER_WAVE * (ET / WAVE);      // rows per following workgroup
+__device__ __forceinline__ bool eigh_replay_follow(double* ws, int m, int mp, int rb, long long spin_limit,
+                                                   double2 (*par)[RP_CHUNK * 64], unsigned long long* shared_p)
+{
+    const int np = m / 2;
+    const double2* rot = reinterpret_cast<const double2*>(ws + rot_record_offset(mp));
+    const int* granks = reinterpret_cast<const int*>(ws + rot_record_offset(mp) + (long long)MAX_SWEEPS * (mp - 1) * mp);
+    double* extra = ws + (long long)S_COUNT * mp * mp + mp;
+    const unsigned long long* progress = reinterpret_cast<const unsigned long long*>(extra + EX_PROGRESS);
+    double* Vout = ws + (long long)S_V * mp * mp;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int s = lane < np ? lane : np - 1;                    // lanes past np mirror the last pair
+    int row[RF_ROWS_PER_WAVE];
+    double top[RF_ROWS_PER_WAVE], bot[RF_ROWS_PER_WAVE];
+#pragma unroll
+    for (int q = 0; q < RF_ROWS_PER_WAVE; ++q) {
+        row[q] = rb * RF_ROWS + RF_ROWS_PER_WAVE * wave + q;
+        top[q] = (row[q] == s) ? 1.0 : 0.0;
+        bot[q] = (row[q] == m - 1 - s) ? 1.0 : 0.0;
+    }
+    auto round_step = [&](double2 rv) {
+#pragma unroll
+        for (int q = 0; q < RF_ROWS_PER_WAVE; ++q) {
+            const double t = fma(rv.x, top[q], -(rv.y * bot[q]));
+            const double u = fma(rv.y, top[q], rv.x * bot[q]);
+            if (np == 1) { top[q] = t; bot[q] = u; continue; }
+            const double up = dpp<0x138, 0xF, 0xF>(t);
+            const double dn = dpp<0x130, 0xF, 0xF>(u);
+            const double b0 = readlane_d(u, 0);
+            const double tl = readlane_d(t, np - 1);
+            top[q] = s == 0 ? t : (s == 1 ? b0 : up);
+            bot[q] = s == np - 1 ? tl : dn;
+        }
+    };
+    bool failed = false;
+    for (int ch = 0;; ++ch) {
+        const unsigned long long need = (unsigned long long)(ch + 1) * RP_CHUNK;
+        if (threadIdx.x == 0) {
+            unsigned long long p = __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            long long spins = 0;
+            while (!(p & PROGRESS_FINAL) && p < need) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > spin_limit) { p = ~0ull; break; }
+                p = __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            *shared_p = p;
+        }
+        __syncthreads();                                        // also: chunk ch - 2's buffer no longer read
+        const unsigned long long p = *shared_p;
+        if (p == ~0ull) { failed = true; break; }
+        const long long avail = (long long)(p & (PROGRESS_FINAL - 1));
+        const int nr = (int)min((long long)RP_CHUNK, avail - (long long)ch * RP_CHUNK);
+        if (nr <= 0) break;                                     // final count reached
+        double2* pc = par[ch & 1];
+        for (int e = threadIdx.x; e < nr * np; e += ET) {
+            const int rr = e / np, ss = e - rr * np;
+            const unsigned long long* src =
+                reinterpret_cast<const unsigned long long*>(rot + (long long)(ch * RP_CHUNK + rr) * np + ss);
+            const unsigned long long cx = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long sx = __hip_atomic_load(src + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            pc[rr * 64 + ss] = make_double2(__longlong_as_double((long long)cx), __longlong_as_double((long long)sx));
+        }
+        __syncthreads();                                        // chunk ch staged
+        if (nr == RP_CHUNK) {
+            double2 rv[RP_CHUNK];
+#pragma unroll
+            for (int rr = 0; rr < RP_CHUNK; ++rr) rv[rr] = pc[rr * 64 + s];
+#pragma unroll
+            for (int rr = 0; rr < RP_CHUNK; ++rr) round_step(rv[rr]);
+        } else {
+            for (int rr = 0; rr < nr; ++rr) round_step(pc[rr * 64 + s]);
+        }
+        if (nr < RP_CHUNK) break;                               // a short chunk is the last one
+    }
+    if (failed) {
+        if (threadIdx.x == 0)
+            __hip_atomic_store(reinterpret_cast<unsigned long long*>(extra + EX_REPLAY_FAILED),
+                               (unsigned long long)__double_as_longlong(1.0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+    }
+#pragma unroll
+    for (int q = 0; q < RF_ROWS_PER_WAVE; ++q) {
+        const int i = row[q];
+        if (i >= mp) continue;
+        double* out = Vout + (long long)i * mp;
+        if (i < m) {
+            if (lane < np) {
+                out[__hip_atomic_load(granks + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = top[q];
+                out[__hip_atomic_load(granks + m - 1 - lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = bot[q];
+            }
+            for (int j = m + lane; j < mp; j += 64) out[j] = 0.0;
+        } else {
+            for (int j = lane; j < mp; j += 64) out[j] = 0.0;
+        }
+    }
+    return true;
+}
 
 // ---------------------------------------------------------------- chained-HVP PCG
 // The preconditioned CG of the Newton step (dual_recommended_hvp.py:141-162, with the HVP of
@@ -1631,7 +1790,8 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
     const Caps caps = a.caps;
     // max_newton = 0 leaves the Newton loop at once with the cap reached, as the reference's
     // `for newton_step in range(MAX_NEWTON)` followed by the cap error does (:118, :191)
-    int status = sweeps < 0 ? ST_EIGH : (caps.max_newton <= 0 ? ST_NEWTON_CAP : ST_OK);
+    int status = (sweeps < 0 || extra[EX_REPLAY_FAILED] != 0.0) ? ST_EIGH
+                                                                : (caps.max_newton <= 0 ? ST_NEWTON_CAP : ST_OK);
     int n_cg = 0, n_bt = 0, n_hvp = 0;
     const double sqrt_m = sqrt((double)m);
 

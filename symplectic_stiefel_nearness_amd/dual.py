@@ -83,11 +83,13 @@ def _workspace(nbytes, device):
     return torch.empty(max(int(nbytes), 8), dtype=torch.uint8, device=device)
 
 
-def _options(split, caps=None):
+def _options(split, caps=None, shared=False):
     """ssn_options for one call: the module constants FEAS_TOL, MAX_NEWTON, MAX_CG and
     MAX_BACKTRACK as they are at call time (the reference reads its module constants the same
     way, dual_recommended_hvp.py:15-16), overridden by the non-None entries of `caps`
-    (keys max_newton, max_cg, max_backtrack, feas_tol)."""
+    (keys max_newton, max_cg, max_backtrack, feas_tol). shared: other solves run on the device
+    at the same time, so the eigensolver takes its two-launch form and leaves CUs free (same
+    results bit for bit)."""
     caps = dict(caps or {})
     bad = set(caps) - {"max_newton", "max_cg", "max_backtrack", "feas_tol"}
     if bad:
@@ -95,10 +97,10 @@ def _options(split, caps=None):
     get = lambda k, d: d if caps.get(k) is None else caps[k]
     return _lib.SsnOptions(int(get("max_newton", MAX_NEWTON)), int(get("max_cg", MAX_CG)),
                            int(get("max_backtrack", MAX_BACKTRACK)), 1 if split else 0,
-                           float(get("feas_tol", FEAS_TOL)))
+                           float(get("feas_tol", FEAS_TOL)), 1 if shared else 0)
 
 
-def _run_solve(T, split, caps=None):
+def _run_solve(T, split, caps=None, shared=False):
     """One ssn_solve_opts call on device T; returns outputs, workspace and stats."""
     lib = _require_ready()
     batch, n, m = _check_targets(T)
@@ -111,7 +113,7 @@ def _run_solve(T, split, caps=None):
     status = torch.zeros(batch, dtype=torch.int32, device=dev)
     counters = torch.zeros((batch, 12), dtype=torch.int32, device=dev)
     ws = _workspace(lib.ssn_workspace_bytes(batch, n, m) if batch else 8, dev)
-    opts = _options(split, caps)
+    opts = _options(split, caps, shared)
     if batch:
         with torch.cuda.device(dev):
             rc = lib.ssn_solve_opts(_stream(), ctypes.byref(opts), T.data_ptr(), T.stride(0), T.stride(1),
@@ -286,7 +288,7 @@ def solve_batches(Ts, in_flight=MAX_IN_FLIGHT, split=False, return_stats=False):
     for i, T in enumerate(Ts):
         s_ = streams[i % len(streams)]
         with torch.cuda.stream(s_):
-            X, omega, iters, _, stats = _run_solve(T, split)
+            X, omega, iters, _, stats = _run_solve(T, split, shared=len(streams) > 1)
         if s_ is not cur:
             for t in (X, omega, iters, stats["status"], stats["counters"]):
                 t.record_stream(cur)                   # freed by the caller after use on cur
@@ -404,7 +406,7 @@ def _solve_host_pipelined(T, return_stats, split, chunks=None, streams=None, dir
     ws = [_workspace(wsb, dev) for _ in comp]          # one per solve stream, reused in order
     for s_ in (h2d, d2h, *comp):
         s_.wait_stream(cur)                            # buffers and inputs are ready
-    opts = _options(split, caps)
+    opts = _options(split, caps, shared=len(comp) > 1)
     for c, (b0, b1) in enumerate(bounds):
         if direct:
             with torch.cuda.stream(h2d):

@@ -93,10 +93,11 @@ def test_default_options_are_the_reference_constants(lib):
     L = _lib.load()
     o = _lib.SsnOptions()
     L.ssn_default_options(ctypes.byref(o))
-    assert (o.max_newton, o.max_cg, o.max_backtrack, o.split, o.feas_tol) == (100, 150, 40, 0, 1e-8)
+    assert (o.max_newton, o.max_cg, o.max_backtrack, o.split, o.feas_tol, o.shared_device) == (100, 150, 40, 0, 1e-8, 0)
     from symplectic_stiefel_nearness_amd import dual
     d = dual._options(False)
-    assert (d.max_newton, d.max_cg, d.max_backtrack, d.split, d.feas_tol) == (100, 150, 40, 0, 1e-8)
+    assert (d.max_newton, d.max_cg, d.max_backtrack, d.split, d.feas_tol, d.shared_device) == (100, 150, 40, 0, 1e-8, 0)
+    assert dual._options(False, shared=True).shared_device == 1
     assert dual._options(True, dict(max_cg=0)).max_cg == 0 and dual._options(True).split == 1
 
 
@@ -106,7 +107,8 @@ def test_solve_opts_rejects_bad_options_before_any_launch(lib):
     args = [None, 0, 0, 1, 16, 16, None, 0, 0, None, 0, 16, None, None, None, None, 0]
     assert L.ssn_solve_opts(None, None, *args) == _lib.SSN_ERR_ARGS
     for field, value in (("max_newton", -1), ("max_cg", -1), ("max_backtrack", 1101), ("split", 2),
-                         ("feas_tol", float("nan")), ("feas_tol", -1.0), ("feas_tol", float("inf"))):
+                         ("feas_tol", float("nan")), ("feas_tol", -1.0), ("feas_tol", float("inf")),
+                         ("shared_device", 2), ("shared_device", -1)):
         o = _lib.SsnOptions()
         L.ssn_default_options(ctypes.byref(o))
         setattr(o, field, value)

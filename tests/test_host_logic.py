@@ -22,7 +22,7 @@ def test_bench_reads_traffic_of_the_timed_launch_shapes():
     16-target launch of the same kernel is a different row."""
     import bench
     shapes = (("void ssn::gram_partial_kernel<6>", "4096 x 64"),
-              ("void ssn::eigh_kernel<3, false, true>", "32768 x 1"),
+              ("void ssn::eigh_coop_kernel<3>", "131072 x 1"),
               ("void ssn::dual_kernel<6>", "32768 x 1"),
               ("void ssn::recover_lds_kernel<6>", "2048 x 64"))
     got = {k: bench.profiled_traffic(k, g) for k, g in shapes}
