@@ -28,6 +28,10 @@ constexpr int SPLIT_S = 5;          // slices per value (39 bits below the group
 constexpr int SPLIT_THREADS = 512;
 constexpr int SPLIT_WAVES = SPLIT_THREADS / WAVE;
 constexpr int SG_ROWS = 32;         // rows per Gram stage (one MFMA K step)
+// LDS length of a slice column (Gram) or row (recovery) of 32 bf16: 80 bytes, so that the 16-byte
+// operand reads of 8 consecutive lanes (8 columns or rows) fall in distinct banks (a 64-byte
+// stride put every other lane on the same banks)
+constexpr int SL_LEN = 40;
 constexpr int NACC = 4;             // accumulators: orders 2, 3, 4, then 5 .. S + 1
 
 __device__ __forceinline__ f4 mfma_bf16(bf16x8 a, bf16x8 b, f4 c) {
@@ -94,7 +98,7 @@ __device__ __forceinline__ void mfma_products(const bf16x8 (&x)[S], const bf16x8
 // |A[:, c]| and |D[:, c]| over the chunk (one shared exponent per column, so that
 // A^T A + D^T D and A^T D share their scale 2^(E_i + E_j)). Pass 2 streams the chunk in
 // stages of 32 rows: every thread cuts 8 rows of one column into S slices and stores
-// them column-major in LDS, [slice][A|D][col][32 rows], so that an MFMA operand is one
+// them column-major in LDS, [slice][A|D][col][32 rows, padded to SL_LEN], so that an MFMA operand is one
 // 16-byte read. The tile group's waves own fixed output tiles (upper tiles of
 // A^T A + D^T D, all tiles of A^T D). The order-4 accumulator of an A^T A + D^T D tile
 // takes 6 products per row, so every 160 rows the accumulators are folded into FP64.
@@ -108,7 +112,7 @@ struct SGShape {
     static constexpr int MAXT = (NTOT + G * SPLIT_WAVES - 1) / (G * SPLIT_WAVES);
     static constexpr int UNITS = 2 * MP * (SG_ROWS / 8);        // (A|D, col, 8-row group)
     static constexpr int UPT = (UNITS + SPLIT_THREADS - 1) / SPLIT_THREADS;
-    static constexpr int SLICE = 2 * MP * SG_ROWS;              // bf16 per slice
+    static constexpr int SLICE = 2 * MP * SL_LEN;               // bf16 per slice
     static constexpr int FOLD = 160 / SG_ROWS;                  // stages between FP64 folds
 };
 
@@ -136,7 +140,19 @@ void split_gram_kernel(const double* __restrict__ T, long long sT_b, long long s
     __shared__ double colmax[4][MP];
     __shared__ double colscale[MP];
 
-    const int chunk = blockIdx.x / G, grp = blockIdx.x % G, b = blockIdx.y;
+    // the G tile groups of a chunk read the same rows: when the grid's rows of workgroups are whole
+    // multiples of 8 G, block x = 8 G u + 8 p + v (v < 8) is group p of chunk 8 u + v, so the groups
+    // of a chunk are 8 blocks apart and run on the same XCD (workgroups go to the 8 XCDs round-robin)
+    // and the later reads of a row hit that XCD's L2
+    int chunk, grp;
+    if (G > 1 && gridDim.x % (8 * G) == 0) {
+        const int x = blockIdx.x, u = x / (8 * G), rem = x - u * 8 * G;
+        grp = rem / 8;
+        chunk = 8 * u + (rem - grp * 8);
+    } else {
+        chunk = blockIdx.x / G; grp = blockIdx.x % G;
+    }
+    const int b = blockIdx.y;
     const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
     const int row_begin = chunk * chunk_rows;
@@ -224,13 +240,13 @@ void split_gram_kernel(const double* __restrict__ T, long long sT_b, long long s
             cut8<S>(pre[k], colscale[c], out);
 #pragma unroll
             for (int a = 0; a < S; ++a)
-                *reinterpret_cast<bf16x8*>(sl + a * Sh::SLICE + (h * MP + c) * SG_ROWS + 8 * g8) = out[a];
+                *reinterpret_cast<bf16x8*>(sl + a * Sh::SLICE + (h * MP + c) * SL_LEN + 8 * g8) = out[a];
         }
     };
     auto frag = [&](int col, bf16x8 (&x)[S]) {
 #pragma unroll
         for (int a = 0; a < S; ++a)
-            x[a] = *reinterpret_cast<const bf16x8*>(sl + a * Sh::SLICE + (col + r) * SG_ROWS + 8 * g);
+            x[a] = *reinterpret_cast<const bf16x8*>(sl + a * Sh::SLICE + (col + r) * SL_LEN + 8 * g);
     };
 
     const int nstages = (row_end - row_begin + SG_ROWS - 1) / SG_ROWS;
@@ -361,7 +377,7 @@ void split_recover_kernel(const double* __restrict__ T, long long sT_b, long lon
 {
     using Sh = SRShape<NT>;
     constexpr int MP = Sh::MP, KS = Sh::KS, NJ = 2 * NT, ROWS = Sh::ROWS;
-    __shared__ __align__(16) __bf16 as_lds[2][S][ROWS * 32];
+    __shared__ __align__(16) __bf16 as_lds[2][S][ROWS * SL_LEN];
     __shared__ double rscale[ROWS];
     __shared__ int rexp[ROWS];
     const int b = blockIdx.y, tid = threadIdx.x;
@@ -417,7 +433,7 @@ void split_recover_kernel(const double* __restrict__ T, long long sT_b, long lon
             cut8<S>(v, rscale[lr], out);
 #pragma unroll
             for (int a = 0; a < S; ++a)
-                *reinterpret_cast<bf16x8*>(&as_lds[buf][a][lr * 32 + 8 * g8]) = out[a];
+                *reinterpret_cast<bf16x8*>(&as_lds[buf][a][lr * SL_LEN + 8 * g8]) = out[a];
         }
     };
 
@@ -429,8 +445,8 @@ void split_recover_kernel(const double* __restrict__ T, long long sT_b, long lon
         bf16x8 x0[S], x1[S];
 #pragma unroll
         for (int a = 0; a < S; ++a) {
-            x0[a] = *reinterpret_cast<const bf16x8*>(&as_lds[buf][a][(rbp * 32 + r) * 32 + 8 * g]);
-            x1[a] = *reinterpret_cast<const bf16x8*>(&as_lds[buf][a][(rbp * 32 + 16 + r) * 32 + 8 * g]);
+            x0[a] = *reinterpret_cast<const bf16x8*>(&as_lds[buf][a][(rbp * 32 + r) * SL_LEN + 8 * g]);
+            x1[a] = *reinterpret_cast<const bf16x8*>(&as_lds[buf][a][(rbp * 32 + 16 + r) * SL_LEN + 8 * g]);
         }
 #pragma unroll
         for (int t = 0; t < Sh::TPW; ++t) {
