@@ -108,7 +108,7 @@ void adjoint_kernel(AdjArgs a)
         int how;
         const int nh = cg_chained<NT, true>(c, lds, [&](int i, int j) { return ((T1[i * MP + j] - T1[j * MP + i]) / 2) / scale; },
                                             Sr, Si, Pe, slot(S_QT), lam_s, a.tol, X, nullptr, how,
-                                            MAX_CG);
+                                            MAX_CG, R, Pp, Hp, U);
         it = how == 0 ? nh - 1 : nh;
         status = how == 1 ? 0 : 1;
     }

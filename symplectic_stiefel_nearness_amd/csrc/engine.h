@@ -1204,6 +1204,8 @@ __device__ __forceinline__ bool eigh_replay_follow(double* ws, int m, int mp, in
 //     element number; the inner products are twice the upper sums.
 // P is exactly symmetric (mirrored inverse product), so its A fragment P[16 I + r][4 kb + q]
 // is read as P[4 kb + q][16 I + r], a coalesced row segment; Q through its transposed copy.
+// At NT = 8 only p fits the LDS (128 x 129 doubles): the residual, the solution and the owned
+// elements of p itself live in the workspace (RXG), read and written once per element pass.
 template <int NT>
 struct ChainCfg {
     static constexpr int MP = 16 * NT;
@@ -1213,11 +1215,14 @@ struct ChainCfg {
     static constexpr int OFF = 48;                  // after the reduction scratch
     static constexpr int OFF_R = OFF + MP * LS;     // residual, then solution, UN doubles each
     static constexpr int OFF_X = OFF_R + UN;
-    static constexpr bool OK = (NT == 2 || NT == 4 || NT == 6);
+    static constexpr bool RXG = NT == 8;            // residual, solution and p's elements in global memory
+    static constexpr bool OK = (NT == 2 || NT == 4 || NT == 6 || NT == 8);
 };
 static_assert(ChainCfg<6>::OFF_X + ChainCfg<6>::UN <= Cfg<6>::OFF_LAM, "chained CG LDS overlaps lambda");
 static_assert(ChainCfg<4>::OFF_X + ChainCfg<4>::UN <= Cfg<4>::OFF_LAM, "chained CG LDS overlaps lambda");
 static_assert(ChainCfg<2>::OFF_X + ChainCfg<2>::UN <= Cfg<2>::OFF_LAM, "chained CG LDS overlaps lambda");
+static_assert(ChainCfg<8>::OFF + ChainCfg<8>::MP * ChainCfg<8>::LS <= Cfg<8>::OFF_LAM, "chained CG LDS overlaps lambda");
+static_assert(2 * ChainCfg<8>::UN <= ChainCfg<8>::MP * ChainCfg<8>::MP, "residual and solution fit one slot");
 
 // Column block J and term mask (bit t: term t) of a wave's chained-HVP items.
 template <int NT>
@@ -1225,6 +1230,8 @@ __device__ __forceinline__ void chain_items(int wave, int& J, int& tmask) {
     if (NT == 6) {
         if (wave < 4) { J = wave; tmask = 3; }
         else { J = 4 + (wave & 1); tmask = 1 << ((wave >> 1) & 1); }
+    } else if (NT == 8) {
+        J = wave; tmask = 3;
     } else if (NT == 4) {
         J = wave & 3; tmask = 1 << (wave >> 2);
     } else {   // NT == 2
@@ -1310,41 +1317,113 @@ __device__ __forceinline__ void chain_term(int t, int J, int r, int q, const dou
     }
 }
 
+// NT = 8: item (t, J) forms U[:, J] = p S_t[:, J] as chain_term does, then Z_t[:, J] = M_t U[:, J]
+// (M_0 = P, M_1 = -Q) in groups of RG row tiles, each group stored to zs (row-major MP x MP), so that
+// only one group of accumulators is live next to U.
+template <int NT, int RG>
+__device__ __forceinline__ void chain_term_staged(int t, int J, int r, int q, const double* M,
+                                                  const double* Sr, const double* Si, const double* P,
+                                                  const double* QT, double* zs) {
+    constexpr int MP = 16 * NT, LS = MP + 1, K4 = 4 * NT;
+    const __amdgpu_buffer_rsrc_t rS = buf_rsrc(t == 0 ? Sr : Si, MP * MP);
+    const int vS = opaque((q * MP + 16 * J + r) * 8);
+    const double* Mp = M + opaque(r * LS + q);
+    d4 accU[NT];
+#pragma unroll
+    for (int I = 0; I < NT; ++I) accU[I] = zero4();
+    double bq[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) bq[u] = buf_ld(rS, vS, 4 * u * MP * 8);
+#pragma unroll 1
+    for (int kb0 = 0; kb0 < K4; kb0 += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kb = kb0 + u;
+            const double bcur = bq[u];
+            bq[u] = buf_ld(rS, vS, 4 * (kb + 4) * MP * 8);   // past the end: 0, unused
+            double a[NT];
+#pragma unroll
+            for (int I = 0; I < NT; ++I) a[I] = Mp[16 * I * LS + 4 * kb];
+#pragma unroll
+            for (int I = 0; I < NT; ++I) accU[I] = mfma(a[I], bcur, accU[I]);
+        }
+    }
+    const __amdgpu_buffer_rsrc_t rA = buf_rsrc(t == 0 ? P : QT, MP * MP);
+    const int vA = opaque((q * MP + r) * 8);
+    constexpr int ks = 4 * MP * 8, is = 16 * 8, D = 4;     // A fragments in flight (4 k-blocks of 4 MFMAs)
+    double* zo = zs + opaque((q * MP + 16 * J + r));
+#pragma unroll
+    for (int g0 = 0; g0 < NT; g0 += RG) {
+        d4 accZ[RG];
+#pragma unroll
+        for (int i = 0; i < RG; ++i) accZ[i] = zero4();
+        double an[D][RG];
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+#pragma unroll
+            for (int i = 0; i < RG; ++i) an[d][i] = buf_ld(rA, vA, d * ks + (g0 + i) * is);
+#pragma unroll
+        for (int kb = 0; kb < K4; ++kb) {
+            const double bu = accU[kb >> 2][kb & 3];
+            const double bb = t == 0 ? bu : -bu;
+#pragma unroll
+            for (int i = 0; i < RG; ++i) accZ[i] = mfma(an[kb % D][i], bb, accZ[i]);
+            if (kb + D < K4) {
+#pragma unroll
+                for (int i = 0; i < RG; ++i) an[kb % D][i] = buf_ld(rA, vA, (kb + D) * ks + (g0 + i) * is);
+            }
+            if (SSN_CG_SCHED_BARRIER) __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int i = 0; i < RG; ++i)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) zo[(16 * (g0 + i) + 4 * e) * MP] = accZ[i][e];
+    }
+}
+
 // PCG on the scaled dual Hessian from the initial residual init(i, j) (i < j). Returns the
 // number of Hessian-vector products; `how` = 0 (curvature not positive), 1 (tolerance met),
 // 2 (iteration cap). ADJ selects the adjoint's rule (stop when |r| <= tol |r0|, none if
 // r0 = 0) instead of the forward's (stop unless |r| > tol). Writes the solution (full skew)
 // to X and, if Z0 is given, the preconditioned initial residual (full skew) to Z0.
+// With K::RXG, rx (2 UN doubles: residual, then solution), pg (UN doubles: p's owned elements)
+// and zs0, zs1 (MP^2 doubles each: the two terms of Z, Z = term 0 + term 1) are free workspace
+// areas that hold what does not fit the LDS.
 template <int NT, bool ADJ, class INIT>
 __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
                                           const double* Sr, const double* Si, const double* P, const double* QT,
                                           const double* lam_s, double tol, double* Xg, double* Z0g, int& how,
-                                          int max_cg)
+                                          int max_cg, double* rx = nullptr, double* pg = nullptr,
+                                          double* zs0 = nullptr, double* zs1 = nullptr)
 {
     using K = ChainCfg<NT>;
     constexpr int MP = K::MP, LS = K::LS, UPT = K::UPT, UN = K::UN;
+    constexpr bool RXG = K::RXG;
     double* M = lds + K::OFF;
-    double* Rl = lds + K::OFF_R;
-    double* Xl = lds + K::OFF_X;
+    double* Rl = RXG ? rx : lds + K::OFF_R;
+    double* Xl = RXG ? rx + UN : lds + K::OFF_X;
     const int tid = c.tid, r = c.r, q = c.q;
     int J, tmask;
     chain_items<NT>(c.wave, J, tmask);
     J = __builtin_amdgcn_readfirstlane(J);
     tmask = __builtin_amdgcn_readfirstlane(tmask);
 
-    double Pd[UPT];
+    // p's owned elements: registers, or the workspace area pg (RXG)
+    double Pd[RXG ? 1 : UPT];
+    auto pd_get = [&](int s, int u) -> double { if constexpr (RXG) return pg[u]; else return Pd[s]; };
+    auto pd_set = [&](int s, int u, double v) { if constexpr (RXG) pg[u] = v; else Pd[s] = v; };
     double rz = 0.0, rr0 = 0.0;
     sync();    // earlier users of this LDS area are done
 #pragma unroll
     for (int s = 0; s < UPT; ++s) {
         const int u = tid + s * DT;
-        Pd[s] = 0.0;
+        if constexpr (!RXG) Pd[s] = 0.0;
         if (u < UN) {
             int i, j;
             upper_ij<MP>(u, i, j);
             const double rv = init(i, j);
             const double z = rv / (lam_s[i] + lam_s[j]);
-            Rl[u] = rv; Xl[u] = 0.0; Pd[s] = z;
+            Rl[u] = rv; Xl[u] = 0.0; pd_set(s, u, z);
             rz = fma(rv, z, rz);
             rr0 = fma(rv, rv, rr0);
             M[i * LS + j] = z; M[j * LS + i] = -z;
@@ -1364,20 +1443,44 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
 #pragma unroll 1
     for (int it = 0; it < max_cg && !(ADJ && how == 1); ++it) {
         ++nh;
-        d4 accZ[NT];
+        if constexpr (RXG) {
+            // wave J: both terms of column block J, staged, then Z[:, J] = term 0 + term 1 into LDS
+            chain_term_staged<NT, 4>(0, J, r, q, M, Sr, Si, P, QT, zs0);
+            chain_term_staged<NT, 4>(1, J, r, q, M, Sr, Si, P, QT, zs1);
+            __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            sync();    // every wave has finished reading p
+            const int lo = opaque(c.lane);
+#pragma unroll
+            for (int g0 = 0; g0 < 32; g0 += 8) {
+                double v0[8], v1[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int e = lo + (g0 + u) * 64, i = e >> 4, j = 16 * J + (e & 15);
+                    v0[u] = zs0[i * MP + j]; v1[u] = zs1[i * MP + j];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int e = lo + (g0 + u) * 64, i = e >> 4, j = 16 * J + (e & 15);
+                    M[i * LS + j] = v0[u] + v1[u];
+                }
+            }
+        }
+        d4 accZ[RXG ? 1 : NT];
+        if constexpr (!RXG) {
 #pragma unroll
         for (int I = 0; I < NT; ++I) accZ[I] = zero4();
         if (tmask & 1) chain_term<NT>(0, J, r, q, M, Sr, Si, P, QT, accZ);
         if (tmask & 2) chain_term<NT>(1, J, r, q, M, Sr, Si, P, QT, accZ);
-        sync();    // every wave has finished reading p
-        if (tmask == 3 || tmask == 1) {
+        }
+        sync();    // every wave has finished reading p (RXG: has written its column block of Z)
+        if (!RXG && (tmask == 3 || tmask == 1)) {
 #pragma unroll
             for (int I = 0; I < NT; ++I)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) M[(16 * I + q + 4 * e) * LS + 16 * J + r] = accZ[I][e];
         }
         sync();
-        if (tmask == 2) {
+        if (!RXG && tmask == 2) {
 #pragma unroll
             for (int I = 0; I < NT; ++I)
 #pragma unroll
@@ -1399,20 +1502,49 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
                 upper_ij<MP>(u, i, j);
                 const double h = M[i * LS + j] - M[j * LS + i];
                 H[s] = h;
-                curv = fma(Pd[s], h, curv);
+                curv = fma(pd_get(s, u), h, curv);
             }
         }
         const double curvature = 2.0 * wg_sum(c, curv);
         if (!(curvature > 0.0)) { how = 0; break; }
         const double alpha = rz / curvature;
         double rr = 0.0, rzn = 0.0;
+        if constexpr (RXG) {
+            // the workspace arrays are distinct, which the compiler cannot tell: the loads of a
+            // group of elements are issued before the group's stores
+            constexpr int GB = 8;
+#pragma unroll
+            for (int g0 = 0; g0 < UPT; g0 += GB) {
+                double xv[GB], rv0[GB], pv0[GB];
+#pragma unroll
+                for (int k = 0; k < GB; ++k) {
+                    const int u = ti + (g0 + k) * DT;
+                    if (g0 + k < UPT && u < UN) { xv[k] = Xl[u]; rv0[k] = Rl[u]; pv0[k] = pg[u]; }
+                }
+#pragma unroll
+                for (int k = 0; k < GB; ++k) {
+                    const int s2 = g0 + k, u = ti + s2 * DT;
+                    if (s2 < UPT && u < UN) {
+                        int i, j;
+                        upper_ij<MP>(u, i, j);
+                        Xl[u] = xv[k] + alpha * pv0[k];
+                        const double rv = rv0[k] - alpha * H[s2];
+                        Rl[u] = rv;
+                        const double z = rv / (lam_s[i] + lam_s[j]);
+                        H[s2] = z;                  // H now holds the preconditioned residual
+                        rr = fma(rv, rv, rr);
+                        rzn = fma(rv, z, rzn);
+                    }
+                }
+            }
+        } else {
 #pragma unroll
         for (int s = 0; s < UPT; ++s) {
             const int u = ti + s * DT;
             if (u < UN) {
                 int i, j;
                 upper_ij<MP>(u, i, j);
-                Xl[u] = Xl[u] + alpha * Pd[s];
+                Xl[u] = Xl[u] + alpha * pd_get(s, u);
                 const double rv = Rl[u] - alpha * H[s];
                 Rl[u] = rv;
                 const double z = rv / (lam_s[i] + lam_s[j]);
@@ -1421,18 +1553,27 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
                 rzn = fma(rv, z, rzn);
             }
         }
+        }
         wg_sum2(c, rr, rzn);    // also: every thread has read Z
         rr *= 2.0; rzn *= 2.0;
         if (ADJ ? (sqrt(rr) <= thr) : !(sqrt(rr) > thr)) { how = 1; break; }
         const double beta = rzn / rz;
+        double pold[RXG ? UPT : 1];
+        if constexpr (RXG) {
+#pragma unroll
+            for (int s = 0; s < UPT; ++s) {
+                const int u = ti + s * DT;
+                pold[s] = u < UN ? pg[u] : 0.0;
+            }
+        }
 #pragma unroll
         for (int s = 0; s < UPT; ++s) {
             const int u = ti + s * DT;
             if (u < UN) {
                 int i, j;
                 upper_ij<MP>(u, i, j);
-                const double pv = H[s] + beta * Pd[s];
-                Pd[s] = pv;
+                const double pv = H[s] + beta * (RXG ? pold[RXG ? s : 0] : pd_get(s, u));
+                pd_set(s, u, pv);
                 M[i * LS + j] = pv; M[j * LS + i] = -pv;
             }
         }
@@ -1525,6 +1666,7 @@ __device__ __forceinline__ void trial_square_lower(const Ctx& c, double* lds) {
 // have the same work, and waves w, w + 4 share a SIMD).
 template <int NT>
 struct QuadCfg {
+    static constexpr bool OK = (NT == 2 || NT == 4 || NT == 6);   // two m x m buffers fit the LDS
     static constexpr int MP = 16 * NT, H = NT / 2, K4 = 4 * NT;
     static constexpr int LY = MP + 1;              // LDS row stride of the two m x m outputs
     static constexpr int OFF = 48;
@@ -1643,6 +1785,225 @@ __device__ __forceinline__ void sandwich_quad(const Ctx& c, double* lds, const d
         G[e] = g;
         gg = fma(g, g, gg);
     }
+}
+
+// ---------------------------------------------------------------- sandwich and line-search tail in strips
+// For NT = 8, where the LDS holds one m x m matrix (128 x 129 doubles) but not the two of the
+// quadrant forms: the same products as sandwich_quad and ls_tail_quad, computed in row strips of
+// the outputs (tile row I, all NT tile columns; wave w takes tile row w), with one intermediate in
+// the LDS and the other in a free workspace slot.
+
+// acc[j] += sum over k of A(16 I + ., k) B(k, col j), K = MP = 16 NT, j < TJ: A fragment read from
+// global at a[(4 kb + q) MP + 16 I + r] (vA), B fragment bfrag(kb, j) times bsign.
+template <int NT, int TJ, class BF>
+__device__ __forceinline__ void row_strip_mma(__amdgpu_buffer_rsrc_t rA, int vA, double bsign, BF bfrag, d4 (&acc)[TJ]) {
+    constexpr int MP = 16 * NT, K4 = 4 * NT, D = 2;
+    // A and B fragments are loaded D k-blocks ahead of their MFMAs
+    double ar[D], br[D][TJ];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        ar[d] = buf_ld(rA, vA, 4 * d * MP * 8);
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) br[d][j] = bfrag(d, j);
+    }
+#pragma unroll
+    for (int kb = 0; kb < K4; ++kb) {
+        const double a = ar[kb % D];
+        double b[TJ];
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) b[j] = bsign * br[kb % D][j];
+        if (kb + D < K4) {
+            ar[kb % D] = buf_ld(rA, vA, 4 * (kb + D) * MP * 8);
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) br[kb % D][j] = bfrag(kb + D, j);
+        }
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) acc[j] = mfma(a, b[j], acc[j]);
+    }
+}
+
+// Q = -W P and its transpose QT (NT = 8, wave w: tile row w). W is exactly skew, so its A fragment
+// W[16 I + r][4 kb + q] is read as -W[4 kb + q][16 I + r], a coalesced row segment: the products
+// are those of the staged form (-W[i][k]) P[k][j] = -(W[i][k] P[k][j]), in the same order.
+template <int NT>
+__device__ __forceinline__ void strip_q(const Ctx& c, const double* W, const double* P, double* Q, double* QT) {
+    constexpr int MP = 16 * NT;
+    const int I = c.wave, r = opaque(c.r), q = opaque(c.q);
+    const int lane_off = (q * MP + r) * 8;
+    const __amdgpu_buffer_rsrc_t rB = buf_rsrc(P, MP * MP);
+    const int vB = opaque(lane_off);
+    d4 acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = zero4();
+    sync();    // W and P are written
+    row_strip_mma<NT, NT>(buf_rsrc(W, MP * MP), opaque(lane_off + 16 * I * 8), 1.0,
+                          [&](int kb, int j) { return buf_ld(rB, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int a = 16 * I + q + 4 * e, col = 16 * j + r;
+            Q[a * MP + col] = acc[j][e];
+            QT[col * MP + a] = acc[j][e];
+        }
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    sync();
+}
+
+// S = N C N^H and the gradient as sandwich_quad, for NT = 8 (wave w: tile row w of each output).
+// Yr^T and then T4^T live in the LDS, Yi^T in the workspace slot Yg, T3^T in the slot Tg.
+template <int NT>
+__device__ __forceinline__ void sandwich_strips(const Ctx& c, double* lds, const double* Cie, const double* P,
+                                                const double* QT, const double* lam_s, int m, const double* Jse,
+                                                double* Sr, double* Si, double* G, double& gg, double* Yg, double* Tg) {
+    constexpr int MP = 16 * NT, LY = MP + 1, PERE = MP * MP / DT;
+    static_assert(NT == DW, "one tile row per wave");
+    double* Yl = lds + 48;
+    const int w = c.wave, r = opaque(c.r), q = opaque(c.q);
+    const int I = w;
+    const int lane_off = (q * MP + r) * 8;
+    const int vA = opaque(lane_off + 16 * I * 8), vB = opaque(lane_off);
+    d4 acc[NT];
+    sync();    // the workspace operands and this LDS area are free
+    // round A: Yr^T = diag(lambda) P - Ci^T Q^T (y = 0, to LDS), Yi^T = Ci^T P + diag(lambda) Q^T (y = 1, to Yg)
+#pragma unroll 1
+    for (int y = 0; y < 2; ++y) {
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[j] = zero4();
+        const __amdgpu_buffer_rsrc_t rB = buf_rsrc(y == 0 ? QT : P, MP * MP);
+        row_strip_mma<NT, NT>(buf_rsrc(Cie, MP * MP), vA, 1.0,
+                          [&](int kb, int j) { return buf_ld(rB, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+        const double* E = y == 0 ? P : QT;
+        // the loads of half a strip before its stores (Yg and E are distinct slots, which the
+        // compiler cannot tell)
+#pragma unroll
+        for (int j0 = 0; j0 < NT; j0 += NT / 2) {
+            double ev[NT / 2][4];
+#pragma unroll
+            for (int j = 0; j < NT / 2; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) ev[j][e] = E[(16 * I + q + 4 * e) * MP + 16 * (j0 + j) + r];
+#pragma unroll
+            for (int j = 0; j < NT / 2; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int a = 16 * I + q + 4 * e, col = 16 * (j0 + j) + r;
+                    const double d = a < m ? lam_s[a] : 0.0;
+                    const double v = acc[j0 + j][e];
+                    if (y == 0) Yl[a * LY + col] = ev[j][e] * d - v;
+                    else Yg[a * MP + col] = v + ev[j][e] * d;
+                }
+        }
+    }
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    sync();
+    // round B: T3^T = P Yr^T + Q Yi^T (to Tg), then T4^T = P Yi^T - Q Yr^T (registers, then LDS)
+    const __amdgpu_buffer_rsrc_t rY = buf_rsrc(Yg, MP * MP);
+    const double* Ylq = Yl + q * LY + r;
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = zero4();
+    row_strip_mma<NT, NT>(buf_rsrc(P, MP * MP), vA, 1.0, [&](int kb, int j) { return Ylq[4 * kb * LY + 16 * j]; }, acc);
+    row_strip_mma<NT, NT>(buf_rsrc(QT, MP * MP), vA, 1.0,
+                      [&](int kb, int j) { return buf_ld(rY, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Tg[(16 * I + q + 4 * e) * MP + 16 * j + r] = acc[j][e];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = zero4();
+    row_strip_mma<NT, NT>(buf_rsrc(P, MP * MP), vA, 1.0,
+                      [&](int kb, int j) { return buf_ld(rY, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+    row_strip_mma<NT, NT>(buf_rsrc(QT, MP * MP), vA, -1.0, [&](int kb, int j) { return Ylq[4 * kb * LY + 16 * j]; }, acc);
+    sync();    // every wave has read Yr^T
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Yl[(16 * I + q + 4 * e) * LY + 16 * j + r] = acc[j][e];
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    sync();
+    // Sr = (T3 + T3^T) / 2, Si = (T4 - T4^T) / 2, G = J - Si, with T3[i][j] = T3^T[j][i]
+    const int t0 = opaque(c.tid);
+    constexpr int GB = 8;                    // elements whose loads precede their stores
+#pragma unroll
+    for (int s0 = 0; s0 < PERE; s0 += GB) {
+        double jv[GB], ta[GB], tb[GB];
+#pragma unroll
+        for (int k = 0; k < GB; ++k) {
+            const int e = t0 + (s0 + k) * DT, i = e / MP, j = e - (e / MP) * MP;
+            jv[k] = Jse[e]; ta[k] = Tg[j * MP + i]; tb[k] = Tg[i * MP + j];
+        }
+#pragma unroll
+        for (int k = 0; k < GB; ++k) {
+            const int e = t0 + (s0 + k) * DT, i = e / MP, j = e - (e / MP) * MP;
+            Sr[e] = (ta[k] + tb[k]) / 2;
+            const double si = (Yl[j * LY + i] - Yl[i * LY + j]) / 2;
+            Si[e] = si;
+            const double g = jv[k] - si;
+            G[e] = g;
+            gg = fma(g, g, gg);
+        }
+    }
+}
+
+// The exact change's tail <P' delta, delta Sr + trial (delta Si)> as ls_tail_quad, for NT = 8:
+// V = delta Si (LDS) and U = delta Sr (slot Ug) in strips, then per strip S = U + trial V in
+// registers and T4 = P' delta in two halves of tile columns, summed against S. Returns this
+// thread's partial sum.
+template <int NT>
+__device__ __forceinline__ double ls_tail_strips(const Ctx& c, double* lds, const double* T1, const double* T2,
+                                                 const double* T3, const double* Sr, const double* Si, double* Ug) {
+    constexpr int MP = 16 * NT, LY = MP + 1, HJ = NT / 2;
+    static_assert(NT == DW, "one tile row per wave");
+    double* Vl = lds + 48;
+    const int w = c.wave, r = opaque(c.r), q = opaque(c.q);
+    const int I = w;
+    const int lane_off = (q * MP + r) * 8;
+    const int vA = opaque(lane_off + 16 * I * 8), vB = opaque(lane_off);
+    const __amdgpu_buffer_rsrc_t rD = buf_rsrc(T3, MP * MP);
+    d4 acc[NT];
+    sync();    // the LDS blocks are consumed, T1..T3 are written
+    // round 1: V = delta Si (y = 0, LDS), U = delta Sr (y = 1, Ug); A = delta^T with the sign on B
+#pragma unroll 1
+    for (int y = 0; y < 2; ++y) {
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[j] = zero4();
+        const __amdgpu_buffer_rsrc_t rS = buf_rsrc(y == 0 ? Si : Sr, MP * MP);
+        row_strip_mma<NT, NT>(rD, vA, -1.0, [&](int kb, int j) { return buf_ld(rS, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int a = 16 * I + q + 4 * e, col = 16 * j + r;
+                if (y == 0) Vl[a * LY + col] = acc[j][e];
+                else Ug[a * MP + col] = acc[j][e];
+            }
+    }
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    sync();
+    // round 2: S = U + trial V (registers), T4 = P' delta against it
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = zero4();
+    const double* Vlq = Vl + q * LY + r;
+    row_strip_mma<NT, NT>(buf_rsrc(T1, MP * MP), vA, -1.0, [&](int kb, int j) { return Vlq[4 * kb * LY + 16 * j]; }, acc);
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[j][e] = Ug[(16 * I + q + 4 * e) * MP + 16 * j + r] + acc[j][e];
+    double tail = 0.0;
+    const __amdgpu_buffer_rsrc_t rP = buf_rsrc(T2, MP * MP);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        d4 acc2[HJ];
+#pragma unroll
+        for (int j = 0; j < HJ; ++j) acc2[j] = zero4();
+        row_strip_mma<NT, HJ>(rP, vA, 1.0, [&](int kb, int j) { return buf_ld(rD, vB, (4 * kb * MP + 16 * (HJ * h + j)) * 8); },
+                          acc2);
+#pragma unroll
+        for (int j = 0; j < HJ; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) tail = fma(acc2[j][e], acc[HJ * h + j][e], tail);
+    }
+    return tail;
 }
 
 // ---------------------------------------------------------------- line-search tail in quadrants
@@ -1808,11 +2169,14 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
                 G[e] = g;
                 gg = fma(g, g, gg);
             });
-        } else if constexpr (ChainCfg<NT>::OK) {
+        } else if constexpr (QuadCfg<NT>::OK) {
             gemm<NT, false, false>(c, lds, W, P, [&](int i, int j, double v) {
                 Q[i * MP + j] = -v; QT[j * MP + i] = -v;
             });
             sandwich_quad<NT>(c, lds, Cie, P, QT, lam_s, m, Jse, Sr, Si, G, gg);
+        } else if constexpr (NT == 8) {
+            strip_q<NT>(c, W, P, Q, QT);
+            sandwich_strips<NT>(c, lds, Cie, P, QT, lam_s, m, Jse, Sr, Si, G, gg, Vv, T5);
         } else {
             gemm<NT, false, false>(c, lds, W, P, [&](int i, int j, double v) {
                 Q[i * MP + j] = -v; QT[j * MP + i] = -v;
@@ -1858,7 +2222,7 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
             const unsigned long long t0 = wall_clock64();
             int how;
             const int nh = cg_chained<NT, false>(c, lds, [&](int i, int j) { return -G[i * MP + j]; },
-                                                 Sr, Si, P, QT, lam_s, tol, Dd, Z0, how, caps.max_cg);
+                                                 Sr, Si, P, QT, lam_s, tol, Dd, Z0, how, caps.max_cg, R, Pp, Hp, U);
             n_hvp += nh; n_cg += nh;
             t_cg += wall_clock64() - t0;
         } else {
@@ -1957,8 +2321,10 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
             if (valid) {
                 inverse_product<NT>(c, lds, T2);                        // P' = (L L^T)^-1
                 double tail = 0.0;
-                if constexpr (ChainCfg<NT>::OK) {
+                if constexpr (QuadCfg<NT>::OK) {
                     tail = ls_tail_quad<NT>(c, lds, T1, T2, T3, Sr, Si);
+                } else if constexpr (NT == 8) {
+                    tail = ls_tail_strips<NT>(c, lds, T1, T2, T3, Sr, Si, Vv);
                 } else {
                     gemm<NT, false, false>(c, lds, T3, Sr, [&](int i, int j, double v) { U[i * MP + j] = v; });
                     gemm<NT, false, false>(c, lds, T3, Si, [&](int i, int j, double v) { Vv[i * MP + j] = v; });
