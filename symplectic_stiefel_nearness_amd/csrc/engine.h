@@ -1850,6 +1850,35 @@ __device__ __forceinline__ void strip_q(const Ctx& c, const double* W, const dou
     sync();
 }
 
+// K = I + trial^2 into the packed lower blocks (NT = 8, wave w: tile row w, from the trial in the
+// workspace slot T1). trial is exactly skew, so its A fragment trial[16 I + r][4 kb + q] is read as
+// -trial[4 kb + q][16 I + r] with the sign carried by the B operand: the products and their order
+// are those of the staged product, so K is bitwise the same. The upper tiles are formed and dropped.
+template <int NT>
+__device__ __forceinline__ void trial_square_strips(const Ctx& c, double* lds, const double* T1) {
+    constexpr int MP = 16 * NT;
+    double* blk = lds + Cfg<NT>::OFF_BLK;
+    const int I = c.wave, r = opaque(c.r), q = opaque(c.q);
+    const int lane_off = (q * MP + r) * 8;
+    const __amdgpu_buffer_rsrc_t rT = buf_rsrc(T1, MP * MP);
+    const int vB = opaque(lane_off);
+    d4 acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = zero4();
+    row_strip_mma<NT, NT>(rT, opaque(lane_off + 16 * I * 8), -1.0,
+                          [&](int kb, int j) { return buf_ld(rT, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        if (j > I) continue;
+        double* o = blk + bidx(I, j) * 256;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int i = q + 4 * e;       // row and column within the block
+            o[i * 16 + r] = ((I == j && i == r) ? 1.0 : 0.0) + acc[j][e];
+        }
+    }
+}
+
 // S = N C N^H and the gradient as sandwich_quad, for NT = 8 (wave w: tile row w of each output).
 // Yr^T and then T4^T live in the LDS, Yi^T in the workspace slot Yg, T3^T in the slot Tg.
 template <int NT>
@@ -2309,6 +2338,8 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
             // K = I + trial^2 into the packed lower blocks
             if constexpr (TrialCfg<NT>::OK) {
                 trial_square_lower<NT>(c, lds);
+            } else if constexpr (NT == 8) {
+                trial_square_strips<NT>(c, lds, T1);
             } else {
                 double* blk = lds + Cf::OFF_BLK;
                 gemm<NT, false, false>(c, lds, T1, T1, [&](int i, int j, double v) {
