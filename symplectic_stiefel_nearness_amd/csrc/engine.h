@@ -694,10 +694,12 @@ __device__ __forceinline__ int bidx(int I, int J) { return I * (I + 1) / 2 + J; 
 
 // One wave: Cholesky of the 16x16 SPD block B (lower part read), replaced in place by
 // the inverse X of its factor (lower, zero above). Lane i < 16 holds row i in registers;
-// values move across lanes with readlane only. Each pivot takes one reciprocal square
-// root (hardware estimate plus two Newton steps) instead of a square root and a divide,
-// and the inverse is right-looking, so the dependent chains stay short. Returns false
-// on a non-positive pivot.
+// values move across lanes with readlane only (to scalar registers: the engine has no vector
+// registers to spare for broadcast copies). Each pivot takes one reciprocal square root
+// (hardware estimate plus two Newton steps) instead of a square root and a divide, and the
+// inverse is right-looking, so the dependent chains stay short. The updates of a pivot run on
+// every lane: lanes i <= j only change entries above the diagonal, which nothing reads, and no
+// lane mask has to be formed. Returns false on a non-positive pivot.
 __device__ __forceinline__ bool diag_factor_invert(double* B, int lane) {
     double row[16];
 #pragma unroll
@@ -715,7 +717,7 @@ __device__ __forceinline__ bool diag_factor_invert(double* B, int lane) {
 #pragma unroll
         for (int k = j + 1; k < 16; ++k) {
             const double lkj = readlane(row[j], k);
-            if (lane > j) row[k] = row[k] - row[j] * lkj;
+            row[k] = row[k] - row[j] * lkj;
         }
     }
     // column `lane` of X = L^-1, right-looking: x_k /= L_kk, then x_i -= L_ik x_k (i > k)
