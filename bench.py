@@ -135,6 +135,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-reps", type=int, default=1)
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-m128", action="store_true",
+                    help="skip the 16 x (16384 x 128) configuration (order 128: FP64, split, forward+backward)")
     ap.add_argument("--no-in-flight", action="store_true",
                     help="skip the batches-in-flight measurement (its launches overlap in a kernel trace)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -338,6 +340,47 @@ def main():
             f"graph {graph_ms:.2f} ms (autotune {tune_s:.1f} s)")
         del Tu, solver_g
 
+    # the second split-precision configuration: 16 targets of 16384 x 128 (n = 8192, k = 64, order
+    # 128), single C calls device in / device out, FP64 and split, and forward+backward with an
+    # N(0, 1) upstream gradient
+    m128 = None
+    if not args.no_m128:
+        T8 = inputs.make_case("n8192_k64", 16).to(dev)
+
+        def calls_ms(fn, reps=5):
+            for _ in range(2):
+                fn()
+            torch.cuda.synchronize()
+            t8 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t8) * 1e3 / reps
+
+        ms8 = calls_ms(lambda: ssn.solve_device(T8, check=False))
+        ms8s = calls_ms(lambda: ssn.solve_device(T8, split=True, check=False))
+        X8, _, _, st8 = ssn.solve_device(T8, check=False, return_stats=True)
+        acc8 = acceptance.summary(acceptance.inspect(T8, X8))
+        acc8["status_ok"] = bool((st8["status"] == 0).all().item())
+        c8 = st8["counters"].double().mean(0).cpu()
+        G8 = inputs.make_gradients(8192, 64, 16).to(dev)
+        T8g = T8.clone().requires_grad_(True)
+
+        def fwd_bwd8(split):
+            T8g.grad = None
+            X8_, _, _ = ssn.project(T8g, split=split)
+            torch.autograd.backward(X8_, G8)
+
+        fb8 = calls_ms(lambda: fwd_bwd8(False))
+        fb8s = calls_ms(lambda: fwd_bwd8(True))
+        m128 = {"targets": 16, "n": 8192, "k": 64, "ms_per_step": ms8, "solves_per_s": 16 / (ms8 * 1e-3),
+                "split_ms_per_step": ms8s, "split_solves_per_s": 16 / (ms8s * 1e-3),
+                "fwd_bwd_ms": fb8, "fwd_bwd_ms_split_forward": fb8s, "acceptance": acc8,
+                "newton_kernel_us_per_target": float(c8[10]) / 100.0,
+                "iterations": {"newton_mean": float(c8[0]), "cg_mean": float(c8[1]), "backtracks_mean": float(c8[2])}}
+        log(f"16 x 16384 x 128: fp64 {ms8:.3f} ms, split {ms8s:.3f} ms, fwd+bwd {fb8:.3f} / {fb8s:.3f} ms")
+        del T8, X8, T8g, G8
+
     # reference protocol boundary: CPU tensor in -> CPU tensors out (PCIe included), with the
     # chunked transfer pipeline (default for this size) and without it
     def host_solve_ms(pipeline):
@@ -433,6 +476,7 @@ def main():
         "bwd_ms": bwd_ms,
         "adjoint": {"cg_mean": adj_it, "all_converged": adj_ok},
         "uneven_128": uneven,
+        "n8192_k64_b16": m128,
         "stage_ms": stages,
         "dual_phase_ms": eigh_ms + newton_ms,
         "forward_bytes_per_step": bytes_fwd,
