@@ -436,6 +436,12 @@ def main():
         if k["traffic"] is not None:
             k["traffic_source"] = (f"{PROFILE.relative_to(ROOT)}: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
                                    f"(factors from a 256 MiB device copy) of {pname}, bytes per launch")
+    for k, (pname, grid) in zip(kernels_split, (("void ssn::split_gram_kernel<6, 2, 5>", f"8192 x {BATCH}"),
+                                                ("void ssn::split_recover_kernel<6, 5>", f"32768 x {BATCH}"))):
+        k["traffic"] = profiled_traffic(pname, grid)
+        if k["traffic"] is not None:
+            k["traffic_source"] = (f"{PROFILE.relative_to(ROOT)}: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
+                                   f"(factors from a 256 MiB device copy) of {pname}, bytes per launch")
     dominant = max(kernels, key=lambda k: k["avg_ms"])
     # the dual engine runs one workgroup (one CU) per target: also state the fraction of the FP64
     # MFMA peak of the CUs the launch occupies (the one-launch eigensolver occupies every CU)

@@ -17,14 +17,16 @@ def test_adjoint_status_check_raises():
 
 def test_bench_reads_traffic_of_the_timed_launch_shapes():
     """bench.py takes each stage kernel's HBM bytes per launch from the committed rocprofv3
-    summary, by kernel and launch shape: every stage kernel of the 64-target step has a row, the
-    value comes from the traffic table (not the trace table of the same shapes), and the
-    16-target launch of the same kernel is a different row."""
+    summary, by kernel and launch shape: every stage kernel of the 64-target step (FP64 and split)
+    has a row, the value comes from the traffic table (not the trace table of the same shapes),
+    and the 16-target launch of the same kernel is a different row."""
     import bench
     shapes = (("void ssn::gram_partial_kernel<6>", "4096 x 64"),
               ("void ssn::eigh_coop_kernel<3>", "131072 x 1"),
               ("void ssn::dual_kernel<6>", "32768 x 1"),
-              ("void ssn::recover_lds_kernel<6>", "2048 x 64"))
+              ("void ssn::recover_lds_kernel<6>", "2048 x 64"),
+              ("void ssn::split_gram_kernel<6, 2, 5>", "8192 x 64"),
+              ("void ssn::split_recover_kernel<6, 5>", "32768 x 64"))
     got = {k: bench.profiled_traffic(k, g) for k, g in shapes}
     assert all(v is not None and v > 0 for v in got.values()), got
     # the recovery reads T (403 MB at 64 targets) and writes X (403 MB): at least that much moves
