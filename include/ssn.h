@@ -70,8 +70,10 @@ int ssn_gram(void* stream, const double* T, long long sT_b, long long sT_r, int 
  * Outputs: omega [batch][m][m] (strides sO_b, ldo) in the original basis; P, Q [batch][mp][mp],
  * the inverse parts (I + i omega)^-1 = P + iQ used by recovery; iters [batch] Newton updates;
  * status [batch]; counters [batch][12] (may be NULL) = Newton updates, CG iterations, backtracks,
- * HVPs, Jacobi sweeps, 1 if the eigenvectors stayed in LDS, then device wall-clock ticks (10 ns) spent in the eigensolver, in the whole
- * setup (scaling, eigensolver, basis change), in CG, in the line search, in the kernel, 0.
+ * HVPs, eigensolver rounds (Jacobi sweeps, or multisection rounds of the tridiagonal eigensolver),
+ * eigensolver form (0 Jacobi with V in global memory, 1 Jacobi with V in LDS, 2 tridiagonal),
+ * then device wall-clock ticks (10 ns) spent in the eigensolver, in the whole setup (scaling,
+ * eigensolver, basis change), in CG, in the line search, in the kernel, 0.
  * Replaces dual_batch (dual_recommended_hvp.py:99-191) with hvp_inner / hvp_outer /
  * hvp_outer_off16 / hvp_outer_diag16 / hvp_sandwich_front / hvp_line_partial / hvp_line_finish /
  * hvp_precondition / hvp_cg_update / hvp_cg_next (hvp_fused.cu:1-420, hvp_fused.py:22-143),

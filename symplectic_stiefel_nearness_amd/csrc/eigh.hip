@@ -3,6 +3,7 @@
 // steps for the uneven-batch plan. Replaces torch.linalg.eigh at
 // src/symplectic_projection/dual_recommended_hvp.py:108.
 #include "engine.h"
+#include "eigh_tri.h"
 #include <cstdlib>
 
 namespace ssn {
@@ -110,10 +111,50 @@ static hipError_t launch_eigh_coop(hipStream_t stream, const EighArgs& e, int ba
     return err;
 }
 
+// ---------------------------------------------------------------- tridiagonal eigensolver
+// Orders m <= 96 (padded order <= 96): eigh_tri_kernel, one workgroup per target, with the
+// Jacobi eigensolver as its per-target fallback. LAUNCH_DECLINED for other orders, or when
+// SSN_EIGH_TRI=0 selects the Jacobi forms (diagnostics).
+static bool tri_enabled() {
+    const char* env = getenv("SSN_EIGH_TRI");
+    return !(env && atoi(env) == 0);
+}
+
+template <int NT>
+static hipError_t launch_tri_nt(hipStream_t stream, const EighArgs& e, int batch, size_t lds_limit) {
+    const size_t lds = (size_t)tri_lds_doubles<NT>(e.m) * 8;
+    if (lds > lds_limit) return LAUNCH_DECLINED;
+    hipError_t err = hipFuncSetAttribute((const void*)eigh_tri_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(eigh_tri_kernel<NT>, dim3(batch), dim3(ET), lds, stream, e);
+    return hipGetLastError();
+}
+
+static hipError_t launch_eigh_tri(hipStream_t stream, const double* Cr, long long sC_b, int ldc, int batch, int m,
+                                  double* ws, size_t lds_limit) {
+    if (!tri_enabled() || m < 2 || batch <= 0) return LAUNCH_DECLINED;
+    EighArgs e;
+    e.Cr = Cr; e.sC_b = sC_b; e.ldc = ldc; e.m = m; e.mp = pad16(m); e.ws = ws; e.v_in_lds = 1;
+    switch (e.mp / 16) {
+        case 1: return launch_tri_nt<1>(stream, e, batch, lds_limit);
+        case 2: return launch_tri_nt<2>(stream, e, batch, lds_limit);
+        case 3: return launch_tri_nt<3>(stream, e, batch, lds_limit);
+        case 4: return launch_tri_nt<4>(stream, e, batch, lds_limit);
+        case 5: return launch_tri_nt<5>(stream, e, batch, lds_limit);
+        case 6: return launch_tri_nt<6>(stream, e, batch, lds_limit);
+        default: return LAUNCH_DECLINED;
+    }
+}
+
 // whole_device: the one-launch form (eigensolver and replay together, every CU for the length
 // of the eigensolver) may be used; otherwise, or when it does not fit, the two launches.
 hipError_t launch_eigh(hipStream_t stream, const double* Cr, long long sC_b, int ldc, int batch,
                               int m, double* ws, size_t lds_limit, bool whole_device) {
+    {
+        const hipError_t te = launch_eigh_tri(stream, Cr, sC_b, ldc, batch, m, ws, lds_limit);
+        if (te != LAUNCH_DECLINED) return te;
+    }
     if (SSN_EIGH_REPLAY) {
         // rotations only in the per-target kernel, then the eigenvectors replayed row by row
         const size_t lds = eigh_lds_doubles(m, false) * 8;

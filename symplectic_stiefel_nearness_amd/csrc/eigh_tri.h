@@ -1,0 +1,486 @@
+// Tridiagonal eigensolver of the Dual solver for orders m <= 96: eigh(Cr / scale) with the
+// eigenvalues ascending, as torch.linalg.eigh returns them at
+// src/symplectic_projection/dual_recommended_hvp.py:108. One 512-thread workgroup per target:
+//
+//  1. Householder reduction to tridiagonal form (LAPACK's dsytd2 reflectors, lower storage) with
+//     the matrix in the registers of four waves: wave w holds rows w + 4 s, lane l columns l and
+//     l + 64, so the reflector's entries at a wave's rows are readlane broadcasts, y = A v is a
+//     register product plus four partials per column through LDS, and the rank-2 update needs no
+//     LDS round trip. Reflector k is formed by the wave holding row k; two barriers per reflector.
+//  2. Eigenvalues of the tridiagonal (d, e) by Sturm-count multisection (dstebz's count with the
+//     pivmin guard), P probes per eigenvalue per round, every eigenvalue bracketed to
+//     2 eps |lambda| + eps ||T||.
+//  3. Eigenvectors of the tridiagonal by the twisted factorisation at each eigenvalue (one lane
+//     per eigenvalue): the forward pivots D+ (kept in LDS) and the backward pivots D- (kept in
+//     the target's workspace), the twist index r minimising |gamma_r| = |D+_r + D-_r - (d_r - lambda)|,
+//     and the vector from z_r = 1 outward.
+//  4. Back-transformation V = H_0 ... H_{m-3} Z with V's columns in registers (same layout as
+//     step 1) and the reflectors read from LDS; no barriers.
+//  5. One Newton-Schulz step V <- V (3 I - V^T V) / 2 on the FP64 MFMA, which removes the loss of
+//     orthogonality of the vectors of close eigenvalues (about eps / gap); written to the
+//     eigenvector slot.
+//
+// A target whose tridiagonal nearly splits (some |e_k| <= 64 eps ||T||), whose reduction is not
+// finite, or whose vectors are further than 1e-8 from orthonormal before the Newton-Schulz step
+// (eigenvalues closer than about 1e-8) is solved by the cyclic Jacobi eigensolver instead, in
+// the same workgroup (eigh_body with V in LDS). Outputs are those of eigh_body: ascending
+// eigenvalues (padding 1.0), V zero padded in slot S_V, scale, a positive count (the
+// multisection rounds; -1 only from the Jacobi fallback without convergence), elapsed ticks,
+// and 2 in the slot where the Jacobi kernel records whether V stayed in LDS (1 from the fallback).
+#pragma once
+#include "engine.h"
+
+namespace ssn {
+
+constexpr double TRI_SAFMIN = 2.2250738585072014e-308;
+constexpr double TRI_ORTHO_TOL = 1e-8;    // largest |V^T V - I| entry the Newton-Schulz step accepts
+constexpr int TRI_MAX_ROUNDS = 128;       // multisection rounds (about 21 are needed at m = 96)
+constexpr int EX_TRI_TICKS = 10;          // extra slots 10..14: device ticks at the end of each phase
+
+template <int NT>
+struct TriCfg {
+    static constexpr int MP = 16 * NT;
+    static constexpr int R = MP / 4;            // rows per thread (i = 4 s + q)
+    static constexpr int THR = 4 * MP;          // threads holding the matrix (NT waves)
+    static constexpr int LZ = MP + 1;           // LDS row stride of Z / V
+    static constexpr int NH = MP >= 3 ? MP - 2 : 1;
+    // LDS layout (doubles): [0, 48) scratch; reflectors (row k = v_k over MP entries); Z;
+    // d, e, e^2, tau, y buffer, lo, hi; 16 misc; multisection probes and counts
+    static constexpr int OFF_H = 48;
+    static constexpr int OFF_Z = OFF_H + NH * MP;
+    static constexpr int OFF_D = OFF_Z + MP * LZ;
+    static constexpr int OFF_E = OFF_D + MP;
+    static constexpr int OFF_E2 = OFF_E + MP;
+    static constexpr int OFF_TAU = OFF_E2 + MP;
+    static constexpr int OFF_P = OFF_TAU + MP;
+    static constexpr int OFF_LO = OFF_P + MP;
+    static constexpr int OFF_HI = OFF_LO + MP;
+    static constexpr int OFF_MISC = OFF_HI + MP;
+    static constexpr int OFF_X = OFF_MISC + 16;     // multisection probes: ET doubles, then ET ints
+    static constexpr int LDS_END = OFF_X + ET + ET / 2;
+};
+
+// 1 / x from the hardware estimate and two Newton steps (within an ulp or two; the quotients of
+// the Sturm counts and of the twisted factorisation need no correct rounding). x != 0.
+__device__ __forceinline__ double rcp_refined(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return r;
+}
+
+// number of eigenvalues of the tridiagonal (d, e^2) below x (dstebz's Sturm count; the pivmin
+// guard keeps every pivot nonzero)
+__device__ __forceinline__ int tri_count(const double* d, const double* e2, int m, double x, double pivmin) {
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    int cnt = q < 0.0 ? 1 : 0;
+    for (int i = 1; i < m; ++i) {
+        q = fma(-e2[i - 1], rcp_refined(q), d[i] - x);
+        if (fabs(q) < pivmin) q = -pivmin;
+        cnt += q < 0.0 ? 1 : 0;
+    }
+    return cnt;
+}
+
+__device__ __forceinline__ double tri_guard(double v, double pivmin) { return fabs(v) < pivmin ? -pivmin : v; }
+
+// Returns false (uniformly over the workgroup) when the target must take the Jacobi eigensolver.
+template <int NT>
+__device__ bool eigh_tri_body(const EighArgs& a, int b)
+{
+    using C = TriCfg<NT>;
+    constexpr int MP = C::MP, R = C::R, LZ = C::LZ;
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int m = a.m;
+    const unsigned long long t0 = wall_clock64();
+    double* ws = a.ws + (long long)b * target_doubles(MP);
+    double* Vout = ws + (long long)S_V * MP * MP;
+    double* Dm = ws + (long long)S_T1 * MP * MP;     // D- of every lane's twisted factorisation
+    double* Mg = ws + (long long)S_T2 * MP * MP;     // 3/2 I - 1/2 V^T V
+    double* lam = ws + (long long)S_COUNT * MP * MP;
+    double* extra = lam + MP;
+    const double* Crb = a.Cr + (long long)b * a.sC_b;
+    double* red = lds;
+    double* Hh = lds + C::OFF_H;
+    double* Z = lds + C::OFF_Z;
+    double* dd = lds + C::OFF_D;
+    double* ee = lds + C::OFF_E;
+    double* e2 = lds + C::OFF_E2;
+    double* tau = lds + C::OFF_TAU;
+    double* pb = lds + C::OFF_P;
+    double* lo = lds + C::OFF_LO;
+    double* hi = lds + C::OFF_HI;
+    double* misc = lds + C::OFF_MISC;
+
+    // scale = max(1, tr(Cr) / m) (dual_recommended_hvp.py:106), as eigh_body forms it
+    if (wave == 0) {
+        double t = 0.0;
+        for (int i = lane; i < m; i += WAVE) t += Crb[(long long)i * a.ldc + i];
+        t = wave_sum(t);
+        if (lane == 0) red[40] = t;
+    }
+    sync();
+    const double scale = fmax(red[40] / m, 1.0);
+
+    // ---- 1. Householder reduction, A in registers (the lower triangle of Cr is read, UPLO='L').
+    // Waves 0..3 hold A: wave w the rows i = w + 4 s (s < RW), lane l the columns l and l + 64.
+    // Row i of a wave is wave-uniform, so the reflector's and w's entries at the wave's rows are
+    // readlane broadcasts of the coalesced copies every lane holds at its own columns.
+    constexpr int RW = MP / 4, NC = MP > 64 ? 2 : 1;
+    const bool hw = wave < 4;
+    double ar[RW][NC];
+#pragma unroll
+    for (int s = 0; s < RW; ++s)
+#pragma unroll
+        for (int cc = 0; cc < NC; ++cc) {
+            const int i = wave + 4 * s, col = lane + 64 * cc;
+            double v = 0.0;
+            if (hw && i < m && col < m) {
+                const int hr = i > col ? i : col, lc = i > col ? col : i;
+                v = Crb[(long long)hr * a.ldc + lc] / scale;
+            }
+            ar[s][cc] = v;
+        }
+    double* part = Z;                    // [4][MP] partial products y = A v (Z is free until step 3)
+#pragma unroll 1
+    for (int k = 0; k < m - 2; ++k) {
+        if (wave == (k & 3)) {
+            // row k (= column k): alpha = A[k][k+1], sigma = sum_{c > k+1} A[k][c]^2
+            const int sk = k >> 2;
+            double row[NC];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                row[cc] = ar[0][cc];
+#pragma unroll
+                for (int s = 1; s < RW; ++s) row[cc] = s == sk ? ar[s][cc] : row[cc];
+            }
+            double sg = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                const int col = lane + 64 * cc;
+                if (col > k + 1 && col < MP) sg = fma(row[cc], row[cc], sg);
+            }
+            sg = wave_sum(sg);
+            const double al = readlane_d(k + 1 < 64 ? row[0] : row[NC - 1], (k + 1) & 63);
+            const double dk = readlane_d(k < 64 ? row[0] : row[NC - 1], k & 63);
+            double t = 0.0, beta = al, scal = 0.0;
+            if (sg != 0.0) {
+                const double nrm = sqrt(fma(al, al, sg));
+                beta = al >= 0.0 ? -nrm : nrm;
+                t = (beta - al) / beta;
+                scal = 1.0 / (al - beta);
+            }
+            double* hk = Hh + k * MP;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                const int col = lane + 64 * cc;
+                if (col < MP) hk[col] = col == k + 1 ? 1.0 : (col > k + 1 ? row[cc] * scal : 0.0);
+            }
+            if (lane == 0) { dd[k] = dk; ee[k] = beta; tau[k] = t; }
+        }
+        sync();
+        // y = A v (partial over the wave's rows); p = tau y; K = tau / 2 p^T v; w = p - K v
+        double vc[NC], tk = 0.0;
+        if (hw) {
+            tk = tau[k];
+            const double* hk = Hh + k * MP;
+            double yp[NC];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                const int col = lane + 64 * cc;
+                vc[cc] = col < MP ? hk[col] : 0.0;
+                yp[cc] = 0.0;
+            }
+#pragma unroll
+            for (int s = 0; s < RW; ++s) {
+                const int i = wave + 4 * s;
+                const double vi = readlane_d(i < 64 ? vc[0] : vc[NC - 1], i & 63);
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) yp[cc] = fma(ar[s][cc], vi, yp[cc]);
+            }
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                const int col = lane + 64 * cc;
+                if (col < MP) part[wave * MP + col] = yp[cc];
+            }
+        }
+        sync();
+        if (hw) {
+            double pc[NC], pv = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                const int col = lane + 64 * cc;
+                double y = 0.0;
+                if (col < MP) y = ((part[col] + part[MP + col]) + part[2 * MP + col]) + part[3 * MP + col];
+                pc[cc] = (col > k && col < MP) ? tk * y : 0.0;
+                pv = fma(pc[cc], vc[cc], pv);
+            }
+            pv = wave_sum(pv);                    // the same value in all four waves
+            const double K = 0.5 * tk * pv;
+            double wc[NC];
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) wc[cc] = pc[cc] - K * vc[cc];
+#pragma unroll
+            for (int s = 0; s < RW; ++s) {
+                const int i = wave + 4 * s;
+                const double vi = readlane_d(i < 64 ? vc[0] : vc[NC - 1], i & 63);
+                const double wi = readlane_d(i < 64 ? wc[0] : wc[NC - 1], i & 63);
+#pragma unroll
+                for (int cc = 0; cc < NC; ++cc) ar[s][cc] = fma(-vi, wc[cc], fma(-wi, vc[cc], ar[s][cc]));
+            }
+        }
+    }
+    const unsigned long long t_red = wall_clock64();
+    if (hw) {
+#pragma unroll
+        for (int s = 0; s < RW; ++s)
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                const int i = wave + 4 * s, col = lane + 64 * cc;
+                if (col == m - 2 && i == m - 2) dd[m - 2] = ar[s][cc];
+                if (col == m - 2 && i == m - 1) ee[m - 2] = ar[s][cc];
+                if (col == m - 1 && i == m - 1) dd[m - 1] = ar[s][cc];
+            }
+    }
+    sync();
+
+    // ---- the tridiagonal's bounds, pivmin and the split test (wave 0)
+    if (wave == 0) {
+        double tn = 0.0, gl = INFINITY, gu = -INFINITY, e2max = 0.0, emin = INFINITY;
+        bool bad = false;
+        for (int i = lane; i < m; i += WAVE) {
+            const double em = i > 0 ? fabs(ee[i - 1]) : 0.0, ep = i < m - 1 ? fabs(ee[i]) : 0.0;
+            const double di = dd[i];
+            tn = fmax(tn, fabs(di) + em + ep);
+            gl = fmin(gl, di - em - ep);
+            gu = fmax(gu, di + em + ep);
+            bad = bad || !isfinite(di);
+            if (i < m - 1) {
+                const double ev = ee[i];
+                e2[i] = ev * ev;
+                e2max = fmax(e2max, ev * ev);
+                emin = fmin(emin, fabs(ev));
+                bad = bad || !isfinite(ev);
+            }
+        }
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            tn = fmax(tn, __shfl_xor(tn, o));
+            gl = fmin(gl, __shfl_xor(gl, o));
+            gu = fmax(gu, __shfl_xor(gu, o));
+            e2max = fmax(e2max, __shfl_xor(e2max, o));
+            emin = fmin(emin, __shfl_xor(emin, o));
+        }
+        const bool anybad = __any(bad);
+        if (lane == 0) {
+            const double pivmin = TRI_SAFMIN * fmax(1.0, e2max);
+            const double pad = 4.0 * JAC_EPS * tn * m + 2.0 * pivmin;
+            misc[0] = gl - pad;
+            misc[1] = gu + pad;
+            misc[2] = pivmin;
+            misc[3] = tn;
+            misc[4] = (!anybad && isfinite(tn) && emin > 64.0 * JAC_EPS * tn) ? 1.0 : 0.0;
+            reinterpret_cast<int*>(misc + 8)[0] = 0;
+            reinterpret_cast<int*>(misc + 8)[1] = 0;
+        }
+    }
+    sync();
+    if (misc[4] == 0.0) return false;
+
+    // ---- 2. eigenvalues by multisection
+    const double pivmin = misc[2], tn = misc[3];
+    double* xb = lds + C::OFF_X;
+    int* cb = reinterpret_cast<int*>(lds + C::OFF_X + ET);
+    int* nd = reinterpret_cast<int*>(misc + 8);
+    const int P = min(16, ET / m);
+    if (tid < m) { lo[tid] = misc[0]; hi[tid] = misc[1]; }
+    const int jv = tid / P, pv = tid - jv * P;
+    const double frac = (double)(pv + 1) / (double)(P + 1);
+    int rounds = 0;
+#pragma unroll 1
+    for (;; ++rounds) {
+        sync();
+        if (rounds > 0 && nd[(rounds - 1) & 1] == 0) break;
+        if (rounds == TRI_MAX_ROUNDS) break;
+        if (tid == 0) nd[rounds & 1] = 0;
+        if (tid < P * m) {
+            const double l = lo[jv], h = hi[jv];
+            const double x = l + (h - l) * frac;
+            xb[tid] = x;
+            cb[tid] = tri_count(dd, e2, m, x, pivmin);
+        }
+        sync();
+        if (tid < m) {
+            double l = lo[tid], h = hi[tid];
+            for (int p = 0; p < P; ++p) {
+                const double x = xb[tid * P + p];
+                if (cb[tid * P + p] <= tid) l = x;
+                else { h = x; break; }
+            }
+            lo[tid] = l; hi[tid] = h;
+            if (h - l > fmax(JAC_EPS * tn, 2.0 * JAC_EPS * fmax(fabs(l), fabs(h)))) nd[rounds & 1] = 1;
+        }
+    }
+
+    const unsigned long long t_bis = wall_clock64();
+    // ---- 3. eigenvectors of the tridiagonal: twisted factorisation, one lane per eigenvalue
+    if (tid < m) {
+        const int j = tid;
+        const double sg = 0.5 * (lo[j] + hi[j]);
+        double dp = tri_guard(dd[0] - sg, pivmin);
+        Z[j] = dp;
+        for (int i = 1; i < m; ++i) {
+            dp = tri_guard(fma(-e2[i - 1], rcp_refined(dp), dd[i] - sg), pivmin);
+            Z[i * LZ + j] = dp;
+        }
+        double dm = tri_guard(dd[m - 1] - sg, pivmin);
+        Dm[(long long)(m - 1) * MP + j] = dm;
+        double best = fabs(Z[(m - 1) * LZ + j] + dm - (dd[m - 1] - sg));
+        int r = m - 1;
+        for (int i = m - 2; i >= 0; --i) {
+            const double di = dd[i] - sg;
+            dm = tri_guard(fma(-e2[i], rcp_refined(dm), di), pivmin);
+            Dm[(long long)i * MP + j] = dm;
+            const double g = fabs(Z[i * LZ + j] + dm - di);
+            if (g < best) { best = g; r = i; }
+        }
+        double nrm = 1.0, zp = 1.0;
+        for (int i = r - 1; i >= 0; --i) {
+            const double zi = -(ee[i] * rcp_refined(Z[i * LZ + j])) * zp;
+            Z[i * LZ + j] = zi;
+            nrm = fma(zi, zi, nrm);
+            zp = zi;
+        }
+        zp = 1.0;
+        for (int i = r; i < m - 1; ++i) {
+            const double zi = -(ee[i] * rcp_refined(Dm[(long long)(i + 1) * MP + j])) * zp;
+            Z[(i + 1) * LZ + j] = zi;
+            nrm = fma(zi, zi, nrm);
+            zp = zi;
+        }
+        Z[r * LZ + j] = 1.0;
+        const double inv = 1.0 / sqrt(nrm);
+        for (int i = 0; i < m; ++i) Z[i * LZ + j] = Z[i * LZ + j] * inv;
+    }
+    sync();
+
+    const unsigned long long t_vec = wall_clock64();
+    // ---- 4. back-transformation V = H_0 ... H_{m-3} Z, columns in registers: the thread of
+    // (column c, quarter q) holds rows i = 4 s + q of column c (NT waves)
+    const bool holder = tid < C::THR;
+    const int c = 16 * wave + (lane & 15), q = lane >> 4;
+    double zv[R];
+    if (holder) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            const int i = 4 * s + q;
+            zv[s] = (i < m && c < m) ? Z[i * LZ + c] : 0.0;
+        }
+#pragma unroll 1
+        for (int k = m - 3; k >= 0; --k) {
+            const double tk = tau[k];
+            const double* hk = Hh + k * MP;
+            double vv[R], y = 0.0;
+#pragma unroll
+            for (int s = 0; s < R; ++s) { vv[s] = hk[4 * s + q]; y = fma(vv[s], zv[s], y); }
+            y += __shfl_xor(y, 16); y += __shfl_xor(y, 32);
+            const double f = tk * y;
+#pragma unroll
+            for (int s = 0; s < R; ++s) zv[s] = zv[s] - f * vv[s];
+        }
+    }
+    sync();
+    if (holder) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) Z[(4 * s + q) * LZ + c] = zv[s];
+    }
+    sync();
+
+    const unsigned long long t_back = wall_clock64();
+    // ---- 5. G = V^T V (upper tiles) on the MFMA; M = 3/2 I - 1/2 G to the workspace
+    const int r16 = lane & 15, q4 = lane >> 4;
+    constexpr int NTU = NT * (NT + 1) / 2;
+    double dev = 0.0;
+    bool nonfinite = false;
+    for (int t = wave; t < NTU; t += EW) {
+        int I = 0, rem = t;
+        while (rem >= NT - I) { rem -= NT - I; ++I; }
+        const int J = I + rem;
+        d4 acc = zero4();
+#pragma unroll 4
+        for (int kb = 0; kb < MP / 4; ++kb) {
+            const int row = 4 * kb + q4;
+            acc = mfma(Z[row * LZ + 16 * I + r16], Z[row * LZ + 16 * J + r16], acc);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int ra = 16 * I + q4 + 4 * i, cb_ = 16 * J + r16;
+            const bool in = ra < m && cb_ < m;
+            const double g = acc[i];
+            const double dv = in ? fabs(g - (ra == cb_ ? 1.0 : 0.0)) : 0.0;
+            nonfinite = nonfinite || !isfinite(g);
+            dev = fmax(dev, dv);
+            const double mv = in ? (ra == cb_ ? 1.5 : 0.0) - 0.5 * g : 0.0;
+            Mg[(long long)ra * MP + cb_] = mv;
+            if (I != J) Mg[(long long)cb_ * MP + ra] = mv;
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) dev = fmax(dev, __shfl_xor(dev, o));
+    nonfinite = __any(nonfinite);
+    if (lane == 0) red[16 + wave] = nonfinite ? INFINITY : dev;
+    sync();
+    double dmax = 0.0;
+#pragma unroll
+    for (int w = 0; w < EW; ++w) dmax = fmax(dmax, red[16 + w]);
+    if (!(dmax <= TRI_ORTHO_TOL)) return false;
+
+    // V' = V M into the eigenvector slot (columns already in ascending eigenvalue order)
+    for (int t = wave; t < NT * NT; t += EW) {
+        const int I = t / NT, J = t - (t / NT) * NT;
+        d4 acc = zero4();
+#pragma unroll 4
+        for (int kb = 0; kb < MP / 4; ++kb)
+            acc = mfma(Z[(16 * I + r16) * LZ + 4 * kb + q4], Mg[(long long)(4 * kb + q4) * MP + 16 * J + r16], acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Vout[(long long)(16 * I + q4 + 4 * i) * MP + 16 * J + r16] = acc[i];
+    }
+    for (int i = tid; i < MP; i += ET) lam[i] = i < m ? 0.5 * (lo[i] + hi[i]) : 1.0;
+    if (tid == 0) {
+        extra[0] = scale;
+        extra[1] = (double)rounds;
+        extra[2] = (double)(wall_clock64() - t0);
+        extra[3] = 2.0;                 // counter 5: 2 = the tridiagonal eigensolver
+        extra[EX_REPLAY_FAILED] = 0.0;
+        // device ticks at the end of each phase (diagnostics): reduction, multisection,
+        // tridiagonal eigenvectors, back-transformation, orthogonalisation
+        extra[EX_TRI_TICKS + 0] = (double)(t_red - t0);
+        extra[EX_TRI_TICKS + 1] = (double)(t_bis - t0);
+        extra[EX_TRI_TICKS + 2] = (double)(t_vec - t0);
+        extra[EX_TRI_TICKS + 3] = (double)(t_back - t0);
+        extra[EX_TRI_TICKS + 4] = (double)(wall_clock64() - t0);
+    }
+    return true;
+}
+
+// LDS of the tridiagonal kernel: its own layout or, for the fallback, the Jacobi kernel with V in
+// LDS, whichever is larger
+template <int NT>
+inline int tri_lds_doubles(int m) {
+    const int jac = 48 + jac_area(m) + (m + 1) * (m + 1);
+    return TriCfg<NT>::LDS_END > jac ? TriCfg<NT>::LDS_END : jac;
+}
+
+template <int NT>
+__global__ __launch_bounds__(ET)
+void eigh_tri_kernel(EighArgs a)
+{
+    if (!eigh_tri_body<NT>(a, blockIdx.x)) {
+        sync();
+        eigh_body<JB_BATCH, true, false>(a, blockIdx.x);
+    }
+}
+
+}  // namespace ssn

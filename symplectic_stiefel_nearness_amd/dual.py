@@ -209,7 +209,8 @@ def solve_device(T, return_stats=False, check=True, split=False, refine=False, *
     """Device-resident solve: T is a float64 HIP tensor [batch, 2n, 2k]; outputs stay on the device.
 
     Returns (X, omega, iterations[, stats]) where stats = dict(status, counters[batch, 12]
-    with columns newton, cg, backtracks, hvps, Jacobi sweeps, 0, then 10 ns device ticks of
+    with columns newton, cg, backtracks, hvps, eigensolver rounds, eigensolver form (0 / 1 Jacobi
+    with V in global memory / LDS, 2 tridiagonal), then 10 ns device ticks of
     eigensolver, setup, CG, line search, whole kernel, 0).
     split=True computes the Gram and the recovery as split BF16 products with FP32
     accumulation reassembled to FP64 (every other step in FP64).

@@ -17,6 +17,13 @@ from symplectic_stiefel_nearness_amd import _lib, inputs
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True)
+def jacobi_eigensolver(monkeypatch):
+    """These are tests of the Jacobi forms: SSN_EIGH_TRI=0 turns the tridiagonal eigensolver off
+    (it would otherwise take every order up to 96)."""
+    monkeypatch.setenv("SSN_EIGH_TRI", "0")
+
+
 def say(*args):
     print(*args, flush=True)
     sys.stdout.flush()
