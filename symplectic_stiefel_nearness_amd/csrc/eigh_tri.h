@@ -7,9 +7,9 @@
 //     l + 64, so the reflector's entries at a wave's rows are readlane broadcasts, y = A v is a
 //     register product plus four partials per column through LDS, and the rank-2 update needs no
 //     LDS round trip. Reflector k is formed by the wave holding row k; two barriers per reflector.
-//  2. Eigenvalues of the tridiagonal (d, e) by Sturm-count multisection (dstebz's count with the
-//     pivmin guard), P probes per eigenvalue per round, every eigenvalue bracketed to
-//     2 eps |lambda| + eps ||T||.
+//  2. Eigenvalues of the tridiagonal (d, e) by Sturm-count multisection (sign changes of the
+//     three-term minor recurrence, rescaled), P probes per eigenvalue per round, every eigenvalue
+//     bracketed to 2 eps |lambda| + eps ||T||.
 //  3. Eigenvectors of the tridiagonal by the twisted factorisation at each eigenvalue (one lane
 //     per eigenvalue): the forward pivots D+ (kept in LDS) and the backward pivots D- (kept in
 //     the target's workspace), the twist index r minimising |gamma_r| = |D+_r + D-_r - (d_r - lambda)|,
@@ -45,18 +45,17 @@ struct TriCfg {
     static constexpr int LZ = MP + 1;           // LDS row stride of Z / V
     static constexpr int NH = MP >= 3 ? MP - 2 : 1;
     // LDS layout (doubles): [0, 48) scratch; reflectors (row k = v_k over MP entries); Z;
-    // d, e, e^2, tau, y buffer, lo, hi; 16 misc; multisection probes and counts
+    // d, e, e^2, tau, lo, hi; 16 misc; multisection probes and counts
     static constexpr int OFF_H = 48;
     static constexpr int OFF_Z = OFF_H + NH * MP;
     static constexpr int OFF_D = OFF_Z + MP * LZ;
     static constexpr int OFF_E = OFF_D + MP;
     static constexpr int OFF_E2 = OFF_E + MP;
     static constexpr int OFF_TAU = OFF_E2 + MP;
-    static constexpr int OFF_P = OFF_TAU + MP;
-    static constexpr int OFF_LO = OFF_P + MP;
+    static constexpr int OFF_LO = OFF_TAU + MP;
     static constexpr int OFF_HI = OFF_LO + MP;
     static constexpr int OFF_MISC = OFF_HI + MP;
-    static constexpr int OFF_X = OFF_MISC + 16;     // multisection probes: ET doubles, then ET ints
+    static constexpr int OFF_X = OFF_MISC + 16;     // multisection probes (P m <= ET): ET doubles, ET ints
     static constexpr int LDS_END = OFF_X + ET + ET / 2;
 };
 
@@ -69,16 +68,35 @@ __device__ __forceinline__ double rcp_refined(double x) {
     return r;
 }
 
-// number of eigenvalues of the tridiagonal (d, e^2) below x (dstebz's Sturm count; the pivmin
-// guard keeps every pivot nonzero)
-__device__ __forceinline__ int tri_count(const double* d, const double* e2, int m, double x, double pivmin) {
-    double q = d[0] - x;
-    if (fabs(q) < pivmin) q = -pivmin;
-    int cnt = q < 0.0 ? 1 : 0;
-    for (int i = 1; i < m; ++i) {
-        q = fma(-e2[i - 1], rcp_refined(q), d[i] - x);
-        if (fabs(q) < pivmin) q = -pivmin;
-        cnt += q < 0.0 ? 1 : 0;
+// Number of eigenvalues of the tridiagonal (d, e^2) below x: the sign changes of the Sturm
+// sequence p_0 = 1, p_1 = d_0 - x, p_{i+1} = (d_i - x) p_i - e_{i-1}^2 p_{i-1} (the leading
+// principal minors of T - x I; a change at i is dstebz's negative pivot q_i = p_i / p_{i-1}).
+// One multiply-add per step on the dependence chain, no division; the pair (p_i, p_{i-1}) is
+// rescaled by a power of two every 4 steps (exact, signs kept). An exact zero counts as positive.
+__device__ __forceinline__ int tri_count(const double* d, const double* e2, int m, double x) {
+    double p1 = d[0] - x, p0 = 1.0;
+    bool neg = p1 < 0.0;
+    int cnt = neg ? 1 : 0;
+    int i = 1;
+    for (; i + 4 <= m; i += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double pn = fma(d[i + u] - x, p1, -(e2[i + u - 1] * p0));
+            p0 = p1; p1 = pn;
+            const bool ng = pn < 0.0;
+            cnt += ng != neg ? 1 : 0;
+            neg = ng;
+        }
+        int ex = 0;
+        (void)frexp(fmax(fabs(p1), fabs(p0)), &ex);
+        p1 = ldexp(p1, -ex); p0 = ldexp(p0, -ex);
+    }
+    for (; i < m; ++i) {
+        const double pn = fma(d[i] - x, p1, -(e2[i - 1] * p0));
+        p0 = p1; p1 = pn;
+        const bool ng = pn < 0.0;
+        cnt += ng != neg ? 1 : 0;
+        neg = ng;
     }
     return cnt;
 }
@@ -109,7 +127,6 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
     double* ee = lds + C::OFF_E;
     double* e2 = lds + C::OFF_E2;
     double* tau = lds + C::OFF_TAU;
-    double* pb = lds + C::OFF_P;
     double* lo = lds + C::OFF_LO;
     double* hi = lds + C::OFF_HI;
     double* misc = lds + C::OFF_MISC;
@@ -127,7 +144,8 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
     // ---- 1. Householder reduction, A in registers (the lower triangle of Cr is read, UPLO='L').
     // Waves 0..3 hold A: wave w the rows i = w + 4 s (s < RW), lane l the columns l and l + 64.
     // Row i of a wave is wave-uniform, so the reflector's and w's entries at the wave's rows are
-    // readlane broadcasts of the coalesced copies every lane holds at its own columns.
+    // readlane broadcasts of the coalesced copies every lane holds at its own columns. Reflector k
+    // is formed by the wave holding row k.
     constexpr int RW = MP / 4, NC = MP > 64 ? 2 : 1;
     const bool hw = wave < 4;
     double ar[RW][NC];
@@ -182,7 +200,7 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
         }
         sync();
         // y = A v (partial over the wave's rows); p = tau y; K = tau / 2 p^T v; w = p - K v
-        double vc[NC], tk = 0.0;
+        double vc[NC], vr[RW], tk = 0.0;
         if (hw) {
             tk = tau[k];
             const double* hk = Hh + k * MP;
@@ -196,9 +214,9 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
 #pragma unroll
             for (int s = 0; s < RW; ++s) {
                 const int i = wave + 4 * s;
-                const double vi = readlane_d(i < 64 ? vc[0] : vc[NC - 1], i & 63);
+                vr[s] = readlane_d(i < 64 ? vc[0] : vc[NC - 1], i & 63);
 #pragma unroll
-                for (int cc = 0; cc < NC; ++cc) yp[cc] = fma(ar[s][cc], vi, yp[cc]);
+                for (int cc = 0; cc < NC; ++cc) yp[cc] = fma(ar[s][cc], vr[s], yp[cc]);
             }
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) {
@@ -225,10 +243,9 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
 #pragma unroll
             for (int s = 0; s < RW; ++s) {
                 const int i = wave + 4 * s;
-                const double vi = readlane_d(i < 64 ? vc[0] : vc[NC - 1], i & 63);
                 const double wi = readlane_d(i < 64 ? wc[0] : wc[NC - 1], i & 63);
 #pragma unroll
-                for (int cc = 0; cc < NC; ++cc) ar[s][cc] = fma(-vi, wc[cc], fma(-wi, vc[cc], ar[s][cc]));
+                for (int cc = 0; cc < NC; ++cc) ar[s][cc] = fma(-vr[s], wc[cc], fma(-wi, vc[cc], ar[s][cc]));
             }
         }
     }
@@ -294,7 +311,8 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
     double* xb = lds + C::OFF_X;
     int* cb = reinterpret_cast<int*>(lds + C::OFF_X + ET);
     int* nd = reinterpret_cast<int*>(misc + 8);
-    const int P = min(16, ET / m);
+    // P probes per eigenvalue, one per thread
+    const int P = max(1, min(16, ET / m));
     if (tid < m) { lo[tid] = misc[0]; hi[tid] = misc[1]; }
     const int jv = tid / P, pv = tid - jv * P;
     const double frac = (double)(pv + 1) / (double)(P + 1);
@@ -309,7 +327,7 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
             const double l = lo[jv], h = hi[jv];
             const double x = l + (h - l) * frac;
             xb[tid] = x;
-            cb[tid] = tri_count(dd, e2, m, x, pivmin);
+            cb[tid] = tri_count(dd, e2, m, x);
         }
         sync();
         if (tid < m) {
@@ -323,7 +341,6 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
             if (h - l > fmax(JAC_EPS * tn, 2.0 * JAC_EPS * fmax(fabs(l), fabs(h)))) nd[rounds & 1] = 1;
         }
     }
-
     const unsigned long long t_bis = wall_clock64();
     // ---- 3. eigenvectors of the tridiagonal: twisted factorisation, one lane per eigenvalue
     if (tid < m) {
@@ -382,9 +399,10 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
         for (int k = m - 3; k >= 0; --k) {
             const double tk = tau[k];
             const double* hk = Hh + k * MP;
-            double vv[R], y = 0.0;
+            double vv[R], ya[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int s = 0; s < R; ++s) { vv[s] = hk[4 * s + q]; y = fma(vv[s], zv[s], y); }
+            for (int s = 0; s < R; ++s) { vv[s] = hk[4 * s + q]; ya[s & 3] = fma(vv[s], zv[s], ya[s & 3]); }
+            double y = (ya[0] + ya[1]) + (ya[2] + ya[3]);
             y += __shfl_xor(y, 16); y += __shfl_xor(y, 32);
             const double f = tk * y;
 #pragma unroll
