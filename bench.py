@@ -420,7 +420,8 @@ def main():
                 "fp64_equivalent_tflops": flops / (ms * 1e-3) / 1e12, "avg_ms": ms}
 
     kernels = [roof_mfma("ssn::gram_partial_kernel<6> (+ gram_reduce_tiles_kernel)", f_gram, gram_ms),
-               roof_mfma("ssn::eigh_coop_kernel<3> (rotations and eigenvector replay in one launch)", f_eigh, eigh_ms),
+               roof_mfma("ssn::eigh_tri_kernel<6> (Householder tridiagonalisation, multisection, twisted "
+                         "vectors, back-transformation, Newton-Schulz step)", f_eigh, eigh_ms),
                roof_mfma("ssn::dual_kernel<6>", fd, newton_ms),
                roof_mfma("ssn::recover_lds_kernel<6>", f_rec, rec_ms)]
     kernels_split = [roof_hbm("ssn::split_gram_kernel<6,2,5> (+ gram_reduce_tiles_kernel)", b_gram, f_gram,
@@ -429,7 +430,7 @@ def main():
                               stages_split["recover"])]
     # HBM traffic per launch of each kernel (the main kernel of a stage) at this workload's grid
     for k, (pname, grid) in zip(kernels, (("void ssn::gram_partial_kernel<6>", f"4096 x {BATCH}"),
-                                          ("void ssn::eigh_coop_kernel<3>", f"{512 * BATCH * 4} x 1"),
+                                          ("void ssn::eigh_tri_kernel<6>", f"{512 * BATCH} x 1"),
                                           ("void ssn::dual_kernel<6>", f"{512 * BATCH} x 1"),
                                           ("void ssn::recover_lds_kernel<6>", f"2048 x {BATCH}"))):
         k["traffic"] = profiled_traffic(pname, grid)

@@ -22,7 +22,7 @@ def test_bench_reads_traffic_of_the_timed_launch_shapes():
     and the 16-target launch of the same kernel is a different row."""
     import bench
     shapes = (("void ssn::gram_partial_kernel<6>", "4096 x 64"),
-              ("void ssn::eigh_coop_kernel<3>", "131072 x 1"),
+              ("void ssn::eigh_tri_kernel<6>", "32768 x 1"),
               ("void ssn::dual_kernel<6>", "32768 x 1"),
               ("void ssn::recover_lds_kernel<6>", "2048 x 64"),
               ("void ssn::split_gram_kernel<6, 2, 5>", "8192 x 64"),
