@@ -112,9 +112,9 @@ static hipError_t launch_eigh_coop(hipStream_t stream, const EighArgs& e, int ba
 }
 
 // ---------------------------------------------------------------- tridiagonal eigensolver
-// Orders m <= 96 (padded order <= 96): eigh_tri_kernel, one workgroup per target, with the
-// Jacobi eigensolver as its per-target fallback. LAUNCH_DECLINED for other orders, or when
-// SSN_EIGH_TRI=0 selects the Jacobi forms (diagnostics).
+// Every order (padded order <= 128): eigh_tri_kernel, one workgroup per target, with the Jacobi
+// eigensolver as its per-target fallback. LAUNCH_DECLINED when the LDS does not fit the device,
+// or when SSN_EIGH_TRI=0 selects the Jacobi forms (diagnostics).
 static bool tri_enabled() {
     const char* env = getenv("SSN_EIGH_TRI");
     return !(env && atoi(env) == 0);
@@ -135,7 +135,7 @@ static hipError_t launch_eigh_tri(hipStream_t stream, const double* Cr, long lon
                                   double* ws, size_t lds_limit) {
     if (!tri_enabled() || m < 2 || batch <= 0) return LAUNCH_DECLINED;
     EighArgs e;
-    e.Cr = Cr; e.sC_b = sC_b; e.ldc = ldc; e.m = m; e.mp = pad16(m); e.ws = ws; e.v_in_lds = 1;
+    e.Cr = Cr; e.sC_b = sC_b; e.ldc = ldc; e.m = m; e.mp = pad16(m); e.ws = ws; e.v_in_lds = e.mp <= 96 ? 1 : 0;
     switch (e.mp / 16) {
         case 1: return launch_tri_nt<1>(stream, e, batch, lds_limit);
         case 2: return launch_tri_nt<2>(stream, e, batch, lds_limit);
@@ -143,6 +143,8 @@ static hipError_t launch_eigh_tri(hipStream_t stream, const double* Cr, long lon
         case 4: return launch_tri_nt<4>(stream, e, batch, lds_limit);
         case 5: return launch_tri_nt<5>(stream, e, batch, lds_limit);
         case 6: return launch_tri_nt<6>(stream, e, batch, lds_limit);
+        case 7: return launch_tri_nt<7>(stream, e, batch, lds_limit);
+        case 8: return launch_tri_nt<8>(stream, e, batch, lds_limit);
         default: return LAUNCH_DECLINED;
     }
 }

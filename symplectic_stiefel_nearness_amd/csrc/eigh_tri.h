@@ -1,4 +1,4 @@
-// Tridiagonal eigensolver of the Dual solver for orders m <= 96: eigh(Cr / scale) with the
+// Tridiagonal eigensolver of the Dual solver for orders m <= 128: eigh(Cr / scale) with the
 // eigenvalues ascending, as torch.linalg.eigh returns them at
 // src/symplectic_projection/dual_recommended_hvp.py:108. One 512-thread workgroup per target:
 //
@@ -14,8 +14,10 @@
 //     per eigenvalue): the forward pivots D+ (kept in LDS) and the backward pivots D- (kept in
 //     the target's workspace), the twist index r minimising |gamma_r| = |D+_r + D-_r - (d_r - lambda)|,
 //     and the vector from z_r = 1 outward.
-//  4. Back-transformation V = H_0 ... H_{m-3} Z with V's columns in registers (same layout as
-//     step 1) and the reflectors read from LDS; no barriers.
+//  4. Back-transformation V = H_0 ... H_{m-3} Z with V's columns in registers (four lanes per
+//     column, rows interleaved by four) and the reflectors read from LDS; no barriers. Above order
+//     96 the reflectors wait in the workspace (LDS holds Z and the current reflector only during
+//     the reduction) and are copied into Z's area once Z is in registers.
 //  5. One Newton-Schulz step V <- V (3 I - V^T V) / 2 on the FP64 MFMA, which removes the loss of
 //     orthogonality of the vectors of close eigenvalues (about eps / gap); written to the
 //     eigenvector slot.
@@ -23,7 +25,7 @@
 // A target whose tridiagonal nearly splits (some |e_k| <= 64 eps ||T||), whose reduction is not
 // finite, or whose vectors are further than 1e-8 from orthonormal before the Newton-Schulz step
 // (eigenvalues closer than about 1e-8) is solved by the cyclic Jacobi eigensolver instead, in
-// the same workgroup (eigh_body with V in LDS). Outputs are those of eigh_body: ascending
+// the same workgroup (eigh_body, V in LDS up to order 96). Outputs are those of eigh_body: ascending
 // eigenvalues (padding 1.0), V zero padded in slot S_V, scale, a positive count (the
 // multisection rounds; -1 only from the Jacobi fallback without convergence), elapsed ticks,
 // and 2 in the slot where the Jacobi kernel records whether V stayed in LDS (1 from the fallback).
@@ -44,10 +46,13 @@ struct TriCfg {
     static constexpr int THR = 4 * MP;          // threads holding the matrix (NT waves)
     static constexpr int LZ = MP + 1;           // LDS row stride of Z / V
     static constexpr int NH = MP >= 3 ? MP - 2 : 1;
-    // LDS layout (doubles): [0, 48) scratch; reflectors (row k = v_k over MP entries); Z;
-    // d, e, e^2, tau, lo, hi; 16 misc; multisection probes and counts
+    // the reflectors stay in LDS up to order 96; above, they go to the workspace (slot S_T3) and
+    // LDS keeps the current one only
+    static constexpr bool H_IN_LDS = NT <= 6;
+    // LDS layout (doubles): [0, 48) scratch; reflectors (row k = v_k over MP entries) or the
+    // current reflector; Z; d, e, e^2, tau, lo, hi; 16 misc; multisection probes and counts
     static constexpr int OFF_H = 48;
-    static constexpr int OFF_Z = OFF_H + NH * MP;
+    static constexpr int OFF_Z = OFF_H + (H_IN_LDS ? NH * MP : MP);
     static constexpr int OFF_D = OFF_Z + MP * LZ;
     static constexpr int OFF_E = OFF_D + MP;
     static constexpr int OFF_E2 = OFF_E + MP;
@@ -117,6 +122,7 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
     double* Vout = ws + (long long)S_V * MP * MP;
     double* Dm = ws + (long long)S_T1 * MP * MP;     // D- of every lane's twisted factorisation
     double* Mg = ws + (long long)S_T2 * MP * MP;     // 3/2 I - 1/2 V^T V
+    double* Hg = ws + (long long)S_T3 * MP * MP;     // the reflectors when they do not stay in LDS
     double* lam = ws + (long long)S_COUNT * MP * MP;
     double* extra = lam + MP;
     const double* Crb = a.Cr + (long long)b * a.sC_b;
@@ -190,20 +196,26 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
                 t = (beta - al) / beta;
                 scal = 1.0 / (al - beta);
             }
-            double* hk = Hh + k * MP;
+            double* hk = C::H_IN_LDS ? Hh + k * MP : Hh;
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) {
                 const int col = lane + 64 * cc;
-                if (col < MP) hk[col] = col == k + 1 ? 1.0 : (col > k + 1 ? row[cc] * scal : 0.0);
+                const double v = col == k + 1 ? 1.0 : (col > k + 1 ? row[cc] * scal : 0.0);
+                if (col < MP) {
+                    hk[col] = v;
+                    if (!C::H_IN_LDS) Hg[(long long)k * MP + col] = v;
+                }
             }
             if (lane == 0) { dd[k] = dk; ee[k] = beta; tau[k] = t; }
         }
         sync();
         // y = A v (partial over the wave's rows); p = tau y; K = tau / 2 p^T v; w = p - K v
-        double vc[NC], vr[RW], tk = 0.0;
+        // v at the wave's rows is kept from y = A v for the update when it fits the registers
+        constexpr bool KEEP_V = RW <= 24;
+        double vc[NC], vr[KEEP_V ? RW : 1], tk = 0.0;
         if (hw) {
             tk = tau[k];
-            const double* hk = Hh + k * MP;
+            const double* hk = C::H_IN_LDS ? Hh + k * MP : Hh;
             double yp[NC];
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) {
@@ -214,9 +226,10 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
 #pragma unroll
             for (int s = 0; s < RW; ++s) {
                 const int i = wave + 4 * s;
-                vr[s] = readlane_d(i < 64 ? vc[0] : vc[NC - 1], i & 63);
+                const double vi = readlane_d(i < 64 ? vc[0] : vc[NC - 1], i & 63);
+                if constexpr (KEEP_V) vr[s] = vi;
 #pragma unroll
-                for (int cc = 0; cc < NC; ++cc) yp[cc] = fma(ar[s][cc], vr[s], yp[cc]);
+                for (int cc = 0; cc < NC; ++cc) yp[cc] = fma(ar[s][cc], vi, yp[cc]);
             }
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) {
@@ -244,8 +257,11 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
             for (int s = 0; s < RW; ++s) {
                 const int i = wave + 4 * s;
                 const double wi = readlane_d(i < 64 ? wc[0] : wc[NC - 1], i & 63);
+                double vi;
+                if constexpr (KEEP_V) vi = vr[s];
+                else vi = readlane_d(i < 64 ? vc[0] : vc[NC - 1], i & 63);
 #pragma unroll
-                for (int cc = 0; cc < NC; ++cc) ar[s][cc] = fma(-vr[s], wc[cc], fma(-wi, vc[cc], ar[s][cc]));
+                for (int cc = 0; cc < NC; ++cc) ar[s][cc] = fma(-vi, wc[cc], fma(-wi, vc[cc], ar[s][cc]));
             }
         }
     }
@@ -395,10 +411,21 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
             const int i = 4 * s + q;
             zv[s] = (i < m && c < m) ? Z[i * LZ + c] : 0.0;
         }
+    }
+    // above order 96 the reflectors come from the workspace into Z's LDS area, free once every
+    // column is in registers ((MP - 2) MP <= MP (MP + 1) doubles)
+    const double* Hb = Hh;
+    if constexpr (!C::H_IN_LDS) {
+        sync();
+        for (int e = tid; e < (m - 2) * MP; e += ET) Z[e] = Hg[e];
+        sync();
+        Hb = Z;
+    }
+    if (holder) {
 #pragma unroll 1
         for (int k = m - 3; k >= 0; --k) {
             const double tk = tau[k];
-            const double* hk = Hh + k * MP;
+            const double* hk = Hb + k * MP;
             double vv[R], ya[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int s = 0; s < R; ++s) { vv[s] = hk[4 * s + q]; ya[s & 3] = fma(vv[s], zv[s], ya[s & 3]); }
@@ -485,9 +512,20 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
 
 // LDS of the tridiagonal kernel: its own layout or, for the fallback, the Jacobi kernel with V in
 // LDS, whichever is larger
+// The Jacobi fallback's form at order m: blocks per update thread within one batch and eigenvector
+// rows per thread within one batch ("small"), V in LDS when it fits (orders up to 96); the same
+// choice as the Jacobi kernel's own launcher makes.
+__host__ __device__ inline bool tri_fallback_small(int m) {
+    const int np = m / 2;
+    const int blocks = (np * (np - 1) / 2 + JU - 1) / JU;
+    const int rows = (m + JU / np - 1) / (JU / np);
+    return blocks <= JB_BATCH && rows <= JV_SMALL;
+}
+
+// LDS of the tridiagonal kernel: its own layout or its Jacobi fallback's, whichever is larger
 template <int NT>
 inline int tri_lds_doubles(int m) {
-    const int jac = 48 + jac_area(m) + (m + 1) * (m + 1);
+    const int jac = 48 + jac_area(m) + (NT <= 6 ? (m + 1) * (m + 1) : 0);
     return TriCfg<NT>::LDS_END > jac ? TriCfg<NT>::LDS_END : jac;
 }
 
@@ -497,7 +535,12 @@ void eigh_tri_kernel(EighArgs a)
 {
     if (!eigh_tri_body<NT>(a, blockIdx.x)) {
         sync();
-        eigh_body<JB_BATCH, true, false>(a, blockIdx.x);
+        if constexpr (NT <= 6) {
+            eigh_body<JB_BATCH, true, false>(a, blockIdx.x);
+        } else {
+            if (tri_fallback_small(a.m)) eigh_body<JB_BATCH, false, false>(a, blockIdx.x);
+            else eigh_body<JB_MAX, false, false>(a, blockIdx.x);
+        }
     }
 }
 
