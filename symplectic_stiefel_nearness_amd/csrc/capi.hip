@@ -45,6 +45,7 @@ hipError_t launch_gram_uneven(hipStream_t, const UTarget*, const UItem*, const i
 hipError_t launch_dual_uneven(hipStream_t, const UTarget*, int, size_t, size_t, int);
 hipError_t launch_recover_uneven(hipStream_t, const UTarget*, const UItem*, int, int, int);
 void eigh_variant(int m, size_t lds_limit, int* small, int* vlds, size_t* lds_bytes);
+size_t eigh_tri_lds_bytes(int m);
 size_t dual_lds_bytes(int m);
 }  // namespace ssn
 
@@ -567,6 +568,7 @@ int ssn_uneven_plan(const ssn_target* targets, int count, void* ws, size_t ws_by
         u.Pf = take(z.pf); u.Qf = take(z.qf); u.frag = take(z.frag);
         size_t el = 0;
         eigh_variant(t.m, lds_limit, &u.eigh_small, &u.eigh_vlds, &el);
+        el = std::max(el, eigh_tri_lds_bytes(t.m));      // either eigensolver form may run
         const size_t dl = dual_lds_bytes(t.m);
         if (el > lds_limit || dl > lds_limit) return SSN_ERR_UNSUPPORTED;
         eigh_lds = std::max(eigh_lds, el);

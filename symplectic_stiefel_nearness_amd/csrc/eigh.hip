@@ -245,8 +245,52 @@ void eigh_variant(int m, size_t lds_limit, int* small, int* vlds, size_t* lds_by
     *lds_bytes = eigh_lds_doubles(m, *vlds != 0) * 8;
 }
 
+// Uneven plans with the tridiagonal eigensolver: one workgroup per target, each running the
+// eigensolver of its own order (its Jacobi fallback included), V formed in the same launch.
+__global__ __launch_bounds__(ET)
+void eigh_tri_uneven_kernel(const UTarget* __restrict__ tg)
+{
+    const UTarget& t = tg[blockIdx.x];
+    EighArgs a;
+    a.Cr = t.Cr; a.sC_b = 0; a.ldc = t.m; a.m = t.m; a.mp = t.mp; a.ws = t.dual; a.v_in_lds = t.mp <= 96 ? 1 : 0;
+    switch (t.mp / 16) {
+        case 1: eigh_tri_target<1>(a, 0); break;
+        case 2: eigh_tri_target<2>(a, 0); break;
+        case 3: eigh_tri_target<3>(a, 0); break;
+        case 4: eigh_tri_target<4>(a, 0); break;
+        case 5: eigh_tri_target<5>(a, 0); break;
+        case 6: eigh_tri_target<6>(a, 0); break;
+        case 7: eigh_tri_target<7>(a, 0); break;
+        case 8: eigh_tri_target<8>(a, 0); break;
+        default: break;
+    }
+}
+
+// dynamic LDS bytes the tridiagonal eigensolver of order m needs (its fallback included)
+size_t eigh_tri_lds_bytes(int m) {
+    switch (pad16(m) / 16) {
+        case 1: return (size_t)tri_lds_doubles<1>(m) * 8;
+        case 2: return (size_t)tri_lds_doubles<2>(m) * 8;
+        case 3: return (size_t)tri_lds_doubles<3>(m) * 8;
+        case 4: return (size_t)tri_lds_doubles<4>(m) * 8;
+        case 5: return (size_t)tri_lds_doubles<5>(m) * 8;
+        case 6: return (size_t)tri_lds_doubles<6>(m) * 8;
+        case 7: return (size_t)tri_lds_doubles<7>(m) * 8;
+        case 8: return (size_t)tri_lds_doubles<8>(m) * 8;
+        default: return 0;
+    }
+}
+
+// eigh_lds: the plan's largest need over both eigensolver forms (eigh_variant, eigh_tri_lds_bytes)
 hipError_t launch_eigh_uneven(hipStream_t stream, const UTarget* tg, int count, size_t eigh_lds)
 {
+    if (tri_enabled()) {
+        hipError_t err = hipFuncSetAttribute((const void*)eigh_tri_uneven_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)eigh_lds);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL(eigh_tri_uneven_kernel, dim3(count), dim3(ET), eigh_lds, stream, tg);
+        return hipGetLastError();
+    }
     hipError_t err = hipFuncSetAttribute((const void*)eigh_uneven_kernel,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)eigh_lds);
     if (err != hipSuccess) return err;

@@ -529,19 +529,26 @@ inline int tri_lds_doubles(int m) {
     return TriCfg<NT>::LDS_END > jac ? TriCfg<NT>::LDS_END : jac;
 }
 
+// Target b's eigensolver: the tridiagonal one, or the Jacobi one when it declines the target.
+template <int NT>
+__device__ __forceinline__ void eigh_tri_target(const EighArgs& a, int b)
+{
+    if (!eigh_tri_body<NT>(a, b)) {
+        sync();
+        if constexpr (NT <= 6) {
+            eigh_body<JB_BATCH, true, false>(a, b);
+        } else {
+            if (tri_fallback_small(a.m)) eigh_body<JB_BATCH, false, false>(a, b);
+            else eigh_body<JB_MAX, false, false>(a, b);
+        }
+    }
+}
+
 template <int NT>
 __global__ __launch_bounds__(ET)
 void eigh_tri_kernel(EighArgs a)
 {
-    if (!eigh_tri_body<NT>(a, blockIdx.x)) {
-        sync();
-        if constexpr (NT <= 6) {
-            eigh_body<JB_BATCH, true, false>(a, blockIdx.x);
-        } else {
-            if (tri_fallback_small(a.m)) eigh_body<JB_BATCH, false, false>(a, blockIdx.x);
-            else eigh_body<JB_MAX, false, false>(a, blockIdx.x);
-        }
-    }
+    eigh_tri_target<NT>(a, blockIdx.x);
 }
 
 }  // namespace ssn
