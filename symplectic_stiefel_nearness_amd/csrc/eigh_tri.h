@@ -14,10 +14,10 @@
 //     per eigenvalue): the forward pivots D+ (kept in LDS) and the backward pivots D- (kept in
 //     the target's workspace), the twist index r minimising |gamma_r| = |D+_r + D-_r - (d_r - lambda)|,
 //     and the vector from z_r = 1 outward.
-//  4. Back-transformation V = H_0 ... H_{m-3} Z with V's columns in registers (four lanes per
-//     column, rows interleaved by four) and the reflectors read from LDS; no barriers. Above order
-//     96 the reflectors wait in the workspace (LDS holds Z and the current reflector only during
-//     the reduction) and are copied into Z's area once Z is in registers.
+//  4. Back-transformation V = H_0 ... H_{m-3} Z on the FP64 MFMA, in compact-WY blocks of 16
+//     reflectors (T factors by dlarft's recurrence), each wave owning whole column tiles of V, so
+//     no barriers between blocks. The reflectors are read from LDS up to order 96 and from the
+//     workspace above (where LDS holds Z and the current reflector only).
 //  5. One Newton-Schulz step V <- V (3 I - V^T V) / 2 on the FP64 MFMA, which removes the loss of
 //     orthogonality of the vectors of close eigenvalues (about eps / gap); written to the
 //     eigenvector slot.
@@ -400,8 +400,88 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
     sync();
 
     const unsigned long long t_vec = wall_clock64();
-    // ---- 4. back-transformation V = H_0 ... H_{m-3} Z, columns in registers: the thread of
-    // (column c, quarter q) holds rows i = 4 s + q of column c (NT waves)
+    // ---- 4. back-transformation V = H_0 ... H_{m-3} Z. Up to order 96 on the MFMA, in place in
+    // Z's LDS area, by blocks of 16 reflectors in compact WY form: H_k0 ... H_k0+15 = I - Y T Y^T (LAPACK's dlarft,
+    // forward, columnwise: T upper triangular). Wave b forms block b's T from S = Y^T Y; then each
+    // wave owns whole column tiles of V and applies the blocks, last first: W1 = Y^T V,
+    // W2 = T W1, V -= Y W2. W1 and W2 stay in the accumulator layout, which is the B-operand
+    // layout of the next product, so no tile goes through LDS, and no barrier is needed
+    // between blocks. (Above order 96 this form read the reflectors from the workspace as MFMA
+    // operands and was slower than the register form below: 102 against 89 us at m = 128.)
+    const int r16 = lane & 15, q4 = lane >> 4;
+    if constexpr (C::H_IN_LDS) {
+    const int nref = m - 2;                                      // reflectors
+    const int nblk = (nref + 15) / 16;
+    double* Tg = ws + (long long)S_T4 * MP * MP;                  // [nblk][16][16] T factors
+    auto yv = [&](int k, int col) -> double {                    // reflector k at column col (0 past the last)
+        if (k >= nref) return 0.0;
+        if constexpr (C::H_IN_LDS) return Hh[k * MP + col];
+        else return Hg[(long long)k * MP + col];
+    };
+    // zero Z's padding (rows or columns past m), which the MFMA products read
+    if (m < MP) {
+        for (int e = tid; e < MP * MP; e += ET) {
+            const int i = e / MP, j = e - (e / MP) * MP;
+            if (i >= m || j >= m) Z[i * LZ + j] = 0.0;
+        }
+    }
+    if (wave < nblk) {
+        const int k0 = 16 * wave;
+        d4 sa = zero4();
+#pragma unroll 4
+        for (int kb = 0; kb < MP / 4; ++kb) {
+            const double y = yv(k0 + r16, 4 * kb + q4);           // S = Y^T Y: A[r][q] = B[q][r]
+            sa = mfma(y, y, sa);
+        }
+        // lane j < 16 forms row j of T: T[j][i] = -tau_i sum_{l=j}^{i-1} T[j][l] S[l][i] (j < i)
+        double trow[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const double ti = k0 + i < nref ? tau[k0 + i] : 0.0;
+            double acc = 0.0;
+#pragma unroll
+            for (int l = 0; l < i; ++l) {
+                const double sli = readlane_d(sa[l >> 2], i + 16 * (l & 3));   // S[l][i]
+                acc = (l >= lane) ? fma(trow[l], sli, acc) : acc;
+            }
+            trow[i] = lane < i ? -ti * acc : (lane == i ? ti : 0.0);
+        }
+        if (lane < 16) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) Tg[(long long)wave * 256 + lane * 16 + i] = trow[i];
+        }
+    }
+    sync();
+    for (int J = wave; J < NT; J += EW) {
+#pragma unroll 1
+        for (int bk = nblk - 1; bk >= 0; --bk) {
+            const int k0 = 16 * bk;
+            d4 w1 = zero4();                                        // W1 = Y^T V[:, J]
+#pragma unroll 4
+            for (int kb = 0; kb < MP / 4; ++kb)
+                w1 = mfma(yv(k0 + r16, 4 * kb + q4), Z[(4 * kb + q4) * LZ + 16 * J + r16], w1);
+            d4 w2 = zero4();                                        // W2 = T W1
+#pragma unroll
+            for (int kb = 0; kb < 4; ++kb)
+                w2 = mfma(Tg[(long long)bk * 256 + r16 * 16 + 4 * kb + q4], w1[kb], w2);
+#pragma unroll 1
+            for (int I = 0; I < NT; ++I) {                          // V[I, J] -= Y[I, :] W2
+                d4 cv;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) cv[i] = Z[(16 * I + q4 + 4 * i) * LZ + 16 * J + r16];
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb) cv = mfma(-yv(k0 + 4 * kb + q4, 16 * I + r16), w2[kb], cv);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) Z[(16 * I + q4 + 4 * i) * LZ + 16 * J + r16] = cv[i];
+            }
+        }
+    }
+    sync();
+
+    } else {
+    // Above order 96 (the reflectors in the workspace): columns in registers, the thread of
+    // (column c, quarter q) holding rows i = 4 s + q of column c (NT waves), the reflectors
+    // copied into Z's LDS area once every column is in registers
     const bool holder = tid < C::THR;
     const int c = 16 * wave + (lane & 15), q = lane >> 4;
     double zv[R];
@@ -443,9 +523,9 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
     }
     sync();
 
+    }
     const unsigned long long t_back = wall_clock64();
     // ---- 5. G = V^T V (upper tiles) on the MFMA; M = 3/2 I - 1/2 G to the workspace
-    const int r16 = lane & 15, q4 = lane >> 4;
     constexpr int NTU = NT * (NT + 1) / 2;
     double dev = 0.0;
     bool nonfinite = false;
