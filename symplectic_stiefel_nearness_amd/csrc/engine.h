@@ -1708,6 +1708,76 @@ __device__ __forceinline__ void quad_mma(__amdgpu_buffer_rsrc_t rA, int vA, doub
     }
 }
 
+// quad_mma with the B fragment read from global as well, at b[(4 kb + q) MP + 16 (J0 + j) + r]
+// (vB holds the lane and J0 part) and scaled by bsign, both operands requested D k-steps ahead.
+template <int NT>
+__device__ __forceinline__ void quad_mma_gg(__amdgpu_buffer_rsrc_t rA, int vA, __amdgpu_buffer_rsrc_t rB, int vB,
+                                            double bsign, d4 (&acc)[QuadCfg<NT>::H][QuadCfg<NT>::H]) {
+    constexpr int MP = QuadCfg<NT>::MP, H = QuadCfg<NT>::H, K4 = QuadCfg<NT>::K4;
+    constexpr int D = 2;
+    double ar[D][H], br[D][H];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            ar[d][i] = buf_ld(rA, vA, (4 * d * MP + 16 * i) * 8);
+            br[d][i] = buf_ld(rB, vB, (4 * d * MP + 16 * i) * 8);
+        }
+#pragma unroll
+    for (int kb = 0; kb < K4; ++kb) {
+        double b[H];
+#pragma unroll
+        for (int j = 0; j < H; ++j) b[j] = bsign * br[kb % D][j];
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+#pragma unroll
+            for (int j = 0; j < H; ++j) acc[i][j] = mfma(ar[kb % D][i], b[j], acc[i][j]);
+        if (kb + D < K4) {
+#pragma unroll
+            for (int i = 0; i < H; ++i) {
+                ar[kb % D][i] = buf_ld(rA, vA, (4 * (kb + D) * MP + 16 * i) * 8);
+                br[kb % D][i] = buf_ld(rB, vB, (4 * (kb + D) * MP + 16 * i) * 8);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// One or two m x m products in quadrants, both operands from global (row-major, MP x MP):
+// C0 = s0 a0^T b0 on waves 0-3 and, when a1 is given, C1 = s1 a1^T b1 on waves 4-7, one quadrant
+// per wave; store(o, i, j, v) receives element (i, j) of output o. The transposed A operand is read
+// as coalesced row segments; a product X B with X symmetric or exactly skew takes a = X (s = 1 or
+// -1), which gives the same products in the same order as the untransposed form. Opens and ends
+// with a barrier.
+template <int NT, class ST>
+__device__ __forceinline__ void quad_gemm2(const Ctx& c, const double* a0, const double* b0, double s0,
+                                           const double* a1, const double* b1, double s1, ST store) {
+    using K = QuadCfg<NT>;
+    constexpr int MP = K::MP, H = K::H;
+    const int w = c.wave, r = opaque(c.r), q = opaque(c.q);
+    const int o = w >> 2, quad = w & 3, I0 = (quad & 1) * H, J0 = ((quad >> 1) & 1) * H;
+    sync();
+    if (o == 0 || a1 != nullptr) {
+        const double* A = o ? a1 : a0;
+        const double* B = o ? b1 : b0;
+        const int lane_off = (q * MP + r) * 8;
+        d4 acc[H][H];
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+#pragma unroll
+            for (int j = 0; j < H; ++j) acc[i][j] = zero4();
+        quad_mma_gg<NT>(buf_rsrc(A, MP * MP), opaque(lane_off + 16 * I0 * 8), buf_rsrc(B, MP * MP),
+                        opaque(lane_off + 16 * J0 * 8), o ? s1 : s0, acc);
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+#pragma unroll
+            for (int j = 0; j < H; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) store(o, 16 * (I0 + i) + q + 4 * e, 16 * (J0 + j) + r, acc[i][j][e]);
+    }
+    sync();
+}
+
 template <int NT>
 __device__ __forceinline__ void sandwich_quad(const Ctx& c, double* lds, const double* Cie, const double* P,
                                               const double* QT, const double* lam_s, int m, const double* Jse,
@@ -1729,8 +1799,7 @@ __device__ __forceinline__ void sandwich_quad(const Ctx& c, double* lds, const d
         const __amdgpu_buffer_rsrc_t rB = buf_rsrc(y == 0 ? QT : P, MP * MP);
         const int vB = opaque(lane_off + 16 * J0 * 8);
         sync();    // the workspace operands and this LDS area are free
-        quad_mma<NT>(buf_rsrc(Cie, MP * MP), opaque(lane_off + 16 * I0 * 8), 1.0,
-                     [&](int kb, int j) { return buf_ld(rB, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+        quad_mma_gg<NT>(buf_rsrc(Cie, MP * MP), opaque(lane_off + 16 * I0 * 8), rB, vB, 1.0, acc);
         const double* E = y == 0 ? P : QT;       // P[a][col] (= P[col][a]) or QT[a][col] (= Q[col][a])
 #pragma unroll
         for (int i = 0; i < H; ++i)
@@ -2067,7 +2136,7 @@ __device__ __forceinline__ double ls_tail_quad(const Ctx& c, double* lds, const 
     sync();    // the LDS blocks are consumed, T1..T3 are written
     {
         const __amdgpu_buffer_rsrc_t rS = buf_rsrc(w < 4 ? Si : Sr, MP * MP);
-        quad_mma<NT>(rD, vA, -1.0, [&](int kb, int j) { return buf_ld(rS, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+        quad_mma_gg<NT>(rD, vA, rS, vB, -1.0, acc);
     }
     if (w < 4) {
 #pragma unroll
@@ -2085,8 +2154,7 @@ __device__ __forceinline__ double ls_tail_quad(const Ctx& c, double* lds, const 
         for (int i = 0; i < H; ++i)
 #pragma unroll
             for (int j = 0; j < H; ++j) acc[i][j] = zero4();
-        quad_mma<NT>(buf_rsrc(T2, MP * MP), vA, 1.0,
-                     [&](int kb, int j) { return buf_ld(rD, vB, (4 * kb * MP + 16 * j) * 8); }, acc);
+        quad_mma_gg<NT>(buf_rsrc(T2, MP * MP), vA, rD, vB, 1.0, acc);
     } else {
         // S = U + trial V
         const double* Vb = Vl + q * LY + 16 * J0 + r;
@@ -2159,14 +2227,26 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
     sync();
 
     // 3. eigenbasis: Ci <- skew(V^T Ci V), J <- skew(V^T J(V)) / scale (:110-111, :53-55)
-    gemm<NT, false, false>(c, lds, T1, V, [&](int i, int j, double v) { T2[i * MP + j] = v; });
-    each<NT>(c, [&](int i, int j, int e) {
-        const int k = m / 2;      // J(V) = [V_bot; -V_top] over the leading m rows
-        T3[e] = (i < k) ? V[(i + k) * MP + j] : (i < m ? -V[(i - k) * MP + j] : 0.0);
-    });
-    sync();
-    gemm<NT, true, false>(c, lds, V, T2, [&](int i, int j, double v) { T4[i * MP + j] = v; });
-    gemm<NT, true, false>(c, lds, V, T3, [&](int i, int j, double v) { T1[i * MP + j] = v; });
+    if constexpr (QuadCfg<NT>::OK) {
+        // T2 = T1 V (T1 = Ci / scale is exactly skew: a = T1, s = -1); then T4 = V^T T2, T1 = V^T T3
+        quad_gemm2<NT>(c, T1, V, -1.0, nullptr, nullptr, 0.0,
+                       [&](int, int i, int j, double v) { T2[i * MP + j] = v; });
+        each<NT>(c, [&](int i, int j, int e) {
+            const int k = m / 2;      // J(V) = [V_bot; -V_top] over the leading m rows
+            T3[e] = (i < k) ? V[(i + k) * MP + j] : (i < m ? -V[(i - k) * MP + j] : 0.0);
+        });
+        quad_gemm2<NT>(c, V, T2, 1.0, V, T3, 1.0,
+                       [&](int o, int i, int j, double v) { (o ? T1 : T4)[i * MP + j] = v; });
+    } else {
+        gemm<NT, false, false>(c, lds, T1, V, [&](int i, int j, double v) { T2[i * MP + j] = v; });
+        each<NT>(c, [&](int i, int j, int e) {
+            const int k = m / 2;      // J(V) = [V_bot; -V_top] over the leading m rows
+            T3[e] = (i < k) ? V[(i + k) * MP + j] : (i < m ? -V[(i - k) * MP + j] : 0.0);
+        });
+        sync();
+        gemm<NT, true, false>(c, lds, V, T2, [&](int i, int j, double v) { T4[i * MP + j] = v; });
+        gemm<NT, true, false>(c, lds, V, T3, [&](int i, int j, double v) { T1[i * MP + j] = v; });
+    }
     each<NT>(c, [&](int i, int j, int e) {
         Cie[e] = (T4[e] - T4[j * MP + i]) / 2;
         Jse[e] = ((T1[e] - T1[j * MP + i]) / 2) / scale;
@@ -2201,8 +2281,9 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
                 gg = fma(g, g, gg);
             });
         } else if constexpr (QuadCfg<NT>::OK) {
-            gemm<NT, false, false>(c, lds, W, P, [&](int i, int j, double v) {
-                Q[i * MP + j] = -v; QT[j * MP + i] = -v;
+            // Q = -W P = W^T P (W exactly skew)
+            quad_gemm2<NT>(c, W, P, 1.0, nullptr, nullptr, 0.0, [&](int, int i, int j, double v) {
+                Q[i * MP + j] = v; QT[j * MP + i] = v;
             });
             sandwich_quad<NT>(c, lds, Cie, P, QT, lam_s, m, Jse, Sr, Si, G, gg);
         } else if constexpr (NT == 8) {
@@ -2385,14 +2466,26 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
     sync();
 
     // 4. back to the original basis (:58-64): omega = skew(V W V^T), P, Q rotated
-    gemm<NT, false, true>(c, lds, W, V, [&](int i, int j, double v) { T1[i * MP + j] = v; });
-    gemm<NT, false, true>(c, lds, P, V, [&](int i, int j, double v) { T2[i * MP + j] = v; });
-    gemm<NT, false, true>(c, lds, Q, V, [&](int i, int j, double v) { T3[i * MP + j] = v; });
     double* Pf = a.Pf + (long long)b * MM;
     double* Qf = a.Qf + (long long)b * MM;
-    gemm<NT, false, false>(c, lds, V, T1, [&](int i, int j, double v) { T4[i * MP + j] = v; });
-    gemm<NT, false, false>(c, lds, V, T2, [&](int i, int j, double v) { Pf[i * MP + j] = v; });
-    gemm<NT, false, false>(c, lds, V, T3, [&](int i, int j, double v) { Qf[i * MP + j] = v; });
+    if constexpr (QuadCfg<NT>::OK) {
+        // V^T into T5; T1 = W V^T (a = W, s = -1: W exactly skew), T2 = P V^T (P exactly
+        // symmetric), T3 = Q V^T (a = Q^T); then T4 = V T1, Pf = V T2, Qf = V T3 (a = V^T)
+        each<NT>(c, [&](int i, int j, int e) { T5[e] = V[j * MP + i]; });
+        quad_gemm2<NT>(c, W, T5, -1.0, P, T5, 1.0,
+                       [&](int o, int i, int j, double v) { (o ? T2 : T1)[i * MP + j] = v; });
+        quad_gemm2<NT>(c, QT, T5, 1.0, T5, T1, 1.0,
+                       [&](int o, int i, int j, double v) { (o ? T4 : T3)[i * MP + j] = v; });
+        quad_gemm2<NT>(c, T5, T2, 1.0, T5, T3, 1.0,
+                       [&](int o, int i, int j, double v) { (o ? Qf : Pf)[i * MP + j] = v; });
+    } else {
+        gemm<NT, false, true>(c, lds, W, V, [&](int i, int j, double v) { T1[i * MP + j] = v; });
+        gemm<NT, false, true>(c, lds, P, V, [&](int i, int j, double v) { T2[i * MP + j] = v; });
+        gemm<NT, false, true>(c, lds, Q, V, [&](int i, int j, double v) { T3[i * MP + j] = v; });
+        gemm<NT, false, false>(c, lds, V, T1, [&](int i, int j, double v) { T4[i * MP + j] = v; });
+        gemm<NT, false, false>(c, lds, V, T2, [&](int i, int j, double v) { Pf[i * MP + j] = v; });
+        gemm<NT, false, false>(c, lds, V, T3, [&](int i, int j, double v) { Qf[i * MP + j] = v; });
+    }
     double* Ob = a.omega + (long long)b * a.sO_b;
     for (int e = c.tid; e < m * m; e += DT) {
         const int i = e / m, j = e - (e / m) * m;
