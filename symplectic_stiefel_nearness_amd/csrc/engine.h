@@ -2183,7 +2183,10 @@ __device__ __forceinline__ double ls_tail_quad(const Ctx& c, double* lds, const 
 
 // ---------------------------------------------------------------- the engine
 // Body of target b's workgroup (DT threads, dynamic LDS Cfg<NT>::LDS_END doubles).
-template <int NT>
+// EXT: the change into the eigenbasis before the iteration (Cie, Jse and the starting W, P, Q,
+// QT in the workspace) and the change back after it (omega, Pf, Qf from W, P, Q) are done by the
+// caller's own launches (dual_tiles_kernel and the element passes in dual.hip), not here.
+template <int NT, bool EXT = false>
 __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
 {
     using Cf = Cfg<NT>;
@@ -2220,6 +2223,7 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
     const unsigned long long t_eigh = (unsigned long long)extra[2];
     const int v_in_lds = (int)extra[3];
     for (int i = c.tid; i < MP; i += DT) lam_s[i] = lam[i];
+    if constexpr (!EXT) {
     // Ci / scale into T1 (dual_recommended_hvp.py:107)
     each<NT>(c, [&](int i, int j, int e) {
         T1[e] = (i < m && j < m) ? Cib[(long long)i * a.ldc + j] / scale : 0.0;
@@ -2256,6 +2260,9 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
         QT[e] = 0.0;
     });
     sync();
+    } else {
+        sync();
+    }
     t_setup = wall_clock64() - t_start;
 
     long long iters = 0;
@@ -2466,6 +2473,7 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
     sync();
 
     // 4. back to the original basis (:58-64): omega = skew(V W V^T), P, Q rotated
+    if constexpr (!EXT) {
     double* Pf = a.Pf + (long long)b * MM;
     double* Qf = a.Qf + (long long)b * MM;
     if constexpr (QuadCfg<NT>::OK) {
@@ -2490,6 +2498,7 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
     for (int e = c.tid; e < m * m; e += DT) {
         const int i = e / m, j = e - (e / m) * m;
         Ob[(long long)i * a.ldo + j] = (T4[i * MP + j] - T4[j * MP + i]) / 2;
+    }
     }
     if (c.tid == 0) {
         a.iters[b] = iters;
