@@ -422,21 +422,31 @@ def main():
     kernels = [roof_mfma("ssn::gram_partial_kernel<6> (+ gram_reduce_tiles_kernel)", f_gram, gram_ms),
                roof_mfma("ssn::eigh_tri_kernel<6> (Householder tridiagonalisation, multisection, twisted "
                          "vectors, back-transformation, Newton-Schulz step)", f_eigh, eigh_ms),
-               roof_mfma("ssn::dual_kernel<6>", fd, newton_ms),
+               roof_mfma("ssn::dual_kernel<6> (+ dual_basis_kernel<6, 0|1>, dual_setup_finish_kernel<6>, "
+                         "dual_omega_kernel<6>: the Newton stage)", fd, newton_ms),
                roof_mfma("ssn::recover_lds_kernel<6>", f_rec, rec_ms)]
     kernels_split = [roof_hbm("ssn::split_gram_kernel<6,2,5> (+ gram_reduce_tiles_kernel)", b_gram, f_gram,
                               stages_split["gram"]),
                      roof_hbm("ssn::split_recover_kernel<6,5> (+ split_recovery_fragments)", b_rec, f_rec,
                               stages_split["recover"])]
     # HBM traffic per launch of each kernel (the main kernel of a stage) at this workload's grid
-    for k, (pname, grid) in zip(kernels, (("void ssn::gram_partial_kernel<6>", f"4096 x {BATCH}"),
-                                          ("void ssn::eigh_tri_kernel<6>", f"{512 * BATCH} x 1"),
-                                          ("void ssn::dual_kernel<6>", f"{512 * BATCH} x 1"),
-                                          ("void ssn::recover_lds_kernel<6>", f"2048 x {BATCH}"))):
-        k["traffic"] = profiled_traffic(pname, grid)
+    # (the Newton stage's traffic is the sum over its five launches, each at its own grid)
+    w = 9216 * BATCH
+    newton_launches = [("void ssn::dual_basis_kernel<6, 0>", f"{w * 2 // 3} x 1"),
+                       ("void ssn::dual_setup_finish_kernel<6>", f"{w} x 1"),
+                       ("void ssn::dual_kernel<6>", f"{512 * BATCH} x 1"),
+                       ("void ssn::dual_basis_kernel<6, 1>", f"{w} x 1"),
+                       ("void ssn::dual_omega_kernel<6>", f"{w} x 1")]
+    for k, launches in zip(kernels, ([("void ssn::gram_partial_kernel<6>", f"4096 x {BATCH}")],
+                                     [("void ssn::eigh_tri_kernel<6>", f"{512 * BATCH} x 1")],
+                                     newton_launches,
+                                     [("void ssn::recover_lds_kernel<6>", f"2048 x {BATCH}")])):
+        parts = [profiled_traffic(pname, grid) for pname, grid in launches]
+        k["traffic"] = None if None in parts else sum(parts)
         if k["traffic"] is not None:
+            names = " + ".join(pname for pname, _ in launches)
             k["traffic_source"] = (f"{PROFILE.relative_to(ROOT)}: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 "
-                                   f"(factors from a 256 MiB device copy) of {pname}, bytes per launch")
+                                   f"(factors from a 256 MiB device copy) of {names}, bytes per launch")
     for k, (pname, grid) in zip(kernels_split, (("void ssn::split_gram_kernel<6, 2, 5>", f"8192 x {BATCH}"),
                                                 ("void ssn::split_recover_kernel<6, 5>", f"32768 x {BATCH}"))):
         k["traffic"] = profiled_traffic(pname, grid)

@@ -24,6 +24,10 @@ def test_bench_reads_traffic_of_the_timed_launch_shapes():
     shapes = (("void ssn::gram_partial_kernel<6>", "4096 x 64"),
               ("void ssn::eigh_tri_kernel<6>", "32768 x 1"),
               ("void ssn::dual_kernel<6>", "32768 x 1"),
+              ("void ssn::dual_basis_kernel<6, 0>", "393216 x 1"),
+              ("void ssn::dual_basis_kernel<6, 1>", "589824 x 1"),
+              ("void ssn::dual_setup_finish_kernel<6>", "589824 x 1"),
+              ("void ssn::dual_omega_kernel<6>", "589824 x 1"),
               ("void ssn::recover_lds_kernel<6>", "2048 x 64"),
               ("void ssn::split_gram_kernel<6, 2, 5>", "8192 x 64"),
               ("void ssn::split_recover_kernel<6, 5>", "32768 x 64"))
