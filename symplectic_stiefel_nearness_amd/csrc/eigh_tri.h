@@ -3,9 +3,9 @@
 // src/symplectic_projection/dual_recommended_hvp.py:108. One 512-thread workgroup per target:
 //
 //  1. Householder reduction to tridiagonal form (LAPACK's dsytd2 reflectors, lower storage) with
-//     the matrix in the registers of four waves: wave w holds rows w + 4 s, lane l columns l and
-//     l + 64, so the reflector's entries at a wave's rows are readlane broadcasts, y = A v is a
-//     register product plus four partials per column through LDS, and the rank-2 update needs no
+//     the matrix in the registers of the eight waves: wave w holds rows w + 8 s, lane l columns l
+//     and l + 64, so the reflector's entries at a wave's rows are readlane broadcasts, y = A v is a
+//     register product plus eight partials per column through LDS, and the rank-2 update needs no
 //     LDS round trip. Reflector k is formed by the wave holding row k; two barriers per reflector.
 //  2. Eigenvalues of the tridiagonal (d, e) by Sturm-count multisection (sign changes of the
 //     three-term minor recurrence, rescaled), P probes per eigenvalue per round, every eigenvalue
@@ -38,6 +38,9 @@ constexpr double TRI_SAFMIN = 2.2250738585072014e-308;
 constexpr double TRI_ORTHO_TOL = 1e-8;    // largest |V^T V - I| entry the Newton-Schulz step accepts
 constexpr int TRI_MAX_ROUNDS = 128;       // multisection rounds (about 21 are needed at m = 96)
 constexpr int EX_TRI_TICKS = 10;          // extra slots 10..14: device ticks at the end of each phase
+#ifndef SSN_TRI_WAVES
+#define SSN_TRI_WAVES 8                   // waves holding the matrix rows during the reduction
+#endif
 
 template <int NT>
 struct TriCfg {
@@ -148,18 +151,20 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
     const double scale = fmax(red[40] / m, 1.0);
 
     // ---- 1. Householder reduction, A in registers (the lower triangle of Cr is read, UPLO='L').
-    // Waves 0..3 hold A: wave w the rows i = w + 4 s (s < RW), lane l the columns l and l + 64.
+    // Waves 0..NW-1 hold A: wave w the rows i = w + NW s (s < RW), lane l the columns l and l + 64.
     // Row i of a wave is wave-uniform, so the reflector's and w's entries at the wave's rows are
     // readlane broadcasts of the coalesced copies every lane holds at its own columns. Reflector k
     // is formed by the wave holding row k.
-    constexpr int RW = MP / 4, NC = MP > 64 ? 2 : 1;
-    const bool hw = wave < 4;
+    constexpr int NW = SSN_TRI_WAVES;
+    static_assert(NW == 4 || NW == 8, "rows are held by 4 or 8 waves");
+    constexpr int RW = MP / NW, NC = MP > 64 ? 2 : 1;
+    const bool hw = wave < NW;
     double ar[RW][NC];
 #pragma unroll
     for (int s = 0; s < RW; ++s)
 #pragma unroll
         for (int cc = 0; cc < NC; ++cc) {
-            const int i = wave + 4 * s, col = lane + 64 * cc;
+            const int i = wave + NW * s, col = lane + 64 * cc;
             double v = 0.0;
             if (hw && i < m && col < m) {
                 const int hr = i > col ? i : col, lc = i > col ? col : i;
@@ -167,12 +172,12 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
             }
             ar[s][cc] = v;
         }
-    double* part = Z;                    // [4][MP] partial products y = A v (Z is free until step 3)
+    double* part = Z;                    // [NW][MP] partial products y = A v (Z is free until step 3)
 #pragma unroll 1
     for (int k = 0; k < m - 2; ++k) {
-        if (wave == (k & 3)) {
+        if (wave == k % NW) {
             // row k (= column k): alpha = A[k][k+1], sigma = sum_{c > k+1} A[k][c]^2
-            const int sk = k >> 2;
+            const int sk = k / NW;
             double row[NC];
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) {
@@ -225,7 +230,7 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
             }
 #pragma unroll
             for (int s = 0; s < RW; ++s) {
-                const int i = wave + 4 * s;
+                const int i = wave + NW * s;
                 const double vi = readlane_d(i < 64 ? vc[0] : vc[NC - 1], i & 63);
                 if constexpr (KEEP_V) vr[s] = vi;
 #pragma unroll
@@ -244,7 +249,11 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
             for (int cc = 0; cc < NC; ++cc) {
                 const int col = lane + 64 * cc;
                 double y = 0.0;
-                if (col < MP) y = ((part[col] + part[MP + col]) + part[2 * MP + col]) + part[3 * MP + col];
+                if (col < MP) {
+                    y = part[col];
+#pragma unroll
+                    for (int u = 1; u < NW; ++u) y += part[u * MP + col];
+                }
                 pc[cc] = (col > k && col < MP) ? tk * y : 0.0;
                 pv = fma(pc[cc], vc[cc], pv);
             }
@@ -255,7 +264,7 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
             for (int cc = 0; cc < NC; ++cc) wc[cc] = pc[cc] - K * vc[cc];
 #pragma unroll
             for (int s = 0; s < RW; ++s) {
-                const int i = wave + 4 * s;
+                const int i = wave + NW * s;
                 const double wi = readlane_d(i < 64 ? wc[0] : wc[NC - 1], i & 63);
                 double vi;
                 if constexpr (KEEP_V) vi = vr[s];
@@ -271,7 +280,7 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
         for (int s = 0; s < RW; ++s)
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) {
-                const int i = wave + 4 * s, col = lane + 64 * cc;
+                const int i = wave + NW * s, col = lane + 64 * cc;
                 if (col == m - 2 && i == m - 2) dd[m - 2] = ar[s][cc];
                 if (col == m - 2 && i == m - 1) ee[m - 2] = ar[s][cc];
                 if (col == m - 1 && i == m - 1) dd[m - 1] = ar[s][cc];
@@ -387,11 +396,23 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
             zp = zi;
         }
         zp = 1.0;
-        for (int i = r; i < m - 1; ++i) {
-            const double zi = -(ee[i] * rcp_refined(Dm[(long long)(i + 1) * MP + j])) * zp;
-            Z[(i + 1) * LZ + j] = zi;
-            nrm = fma(zi, zi, nrm);
-            zp = zi;
+        // D- comes from the workspace: a batch of PB pivots is requested before its steps, so
+        // the loads' latency is paid once per batch rather than once per step
+        constexpr int PB = 16;
+        for (int i0 = r; i0 < m - 1; i0 += PB) {
+            double dv[PB];
+#pragma unroll
+            for (int u = 0; u < PB; ++u) dv[u] = i0 + u < m - 1 ? Dm[(long long)(i0 + u + 1) * MP + j] : 1.0;
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                const int i = i0 + u;
+                if (i < m - 1) {
+                    const double zi = -(ee[i] * rcp_refined(dv[u])) * zp;
+                    Z[(i + 1) * LZ + j] = zi;
+                    nrm = fma(zi, zi, nrm);
+                    zp = zi;
+                }
+            }
         }
         Z[r * LZ + j] = 1.0;
         const double inv = 1.0 / sqrt(nrm);
