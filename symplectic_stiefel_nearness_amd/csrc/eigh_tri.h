@@ -337,15 +337,32 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
     int* cb = reinterpret_cast<int*>(lds + C::OFF_X + ET);
     int* nd = reinterpret_cast<int*>(misc + 8);
     // P probes per eigenvalue, one per thread
-    const int P = max(1, min(16, ET / m));
-    if (tid < m) { lo[tid] = misc[0]; hi[tid] = misc[1]; }
+    const int P = max(1, min(16, ET / m)), PM = P * m;
+    // first round: the P m probes spread evenly over the whole bracket [misc[0], misc[1]]; each
+    // eigenvalue j then starts from the probes around the first one whose count exceeds j
+    if (tid < PM) {
+        const double x = misc[0] + (misc[1] - misc[0]) * ((double)(tid + 1) / (double)(PM + 1));
+        xb[tid] = x;
+        cb[tid] = tri_count(dd, e2, m, x);
+    }
+    sync();
+    if (tid < m) {
+        // binary search: ends with cb[t - 1] <= tid < cb[t] (t = 0: none below; t = PM: none above)
+        int t0 = 0, t1 = PM;
+        while (t0 < t1) {
+            const int mid = (t0 + t1) >> 1;
+            if (cb[mid] > tid) t1 = mid; else t0 = mid + 1;
+        }
+        lo[tid] = t0 > 0 ? xb[t0 - 1] : misc[0];
+        hi[tid] = t0 < PM ? xb[t0] : misc[1];
+    }
     const int jv = tid / P, pv = tid - jv * P;
     const double frac = (double)(pv + 1) / (double)(P + 1);
-    int rounds = 0;
+    int rounds = 1;
 #pragma unroll 1
     for (;; ++rounds) {
         sync();
-        if (rounds > 0 && nd[(rounds - 1) & 1] == 0) break;
+        if (rounds > 1 && nd[(rounds - 1) & 1] == 0) break;
         if (rounds == TRI_MAX_ROUNDS) break;
         if (tid == 0) nd[rounds & 1] = 0;
         if (tid < P * m) {
@@ -367,56 +384,95 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
         }
     }
     const unsigned long long t_bis = wall_clock64();
-    // ---- 3. eigenvectors of the tridiagonal: twisted factorisation, one lane per eigenvalue
-    if (tid < m) {
-        const int j = tid;
-        const double sg = 0.5 * (lo[j] + hi[j]);
-        double dp = tri_guard(dd[0] - sg, pivmin);
-        Z[j] = dp;
-        for (int i = 1; i < m; ++i) {
-            dp = tri_guard(fma(-e2[i - 1], rcp_refined(dp), dd[i] - sg), pivmin);
-            Z[i * LZ + j] = dp;
-        }
-        double dm = tri_guard(dd[m - 1] - sg, pivmin);
-        Dm[(long long)(m - 1) * MP + j] = dm;
-        double best = fabs(Z[(m - 1) * LZ + j] + dm - (dd[m - 1] - sg));
-        int r = m - 1;
-        for (int i = m - 2; i >= 0; --i) {
-            const double di = dd[i] - sg;
-            dm = tri_guard(fma(-e2[i], rcp_refined(dm), di), pivmin);
-            Dm[(long long)i * MP + j] = dm;
-            const double g = fabs(Z[i * LZ + j] + dm - di);
-            if (g < best) { best = g; r = i; }
-        }
-        double nrm = 1.0, zp = 1.0;
-        for (int i = r - 1; i >= 0; --i) {
-            const double zi = -(ee[i] * rcp_refined(Z[i * LZ + j])) * zp;
-            Z[i * LZ + j] = zi;
-            nrm = fma(zi, zi, nrm);
-            zp = zi;
-        }
-        zp = 1.0;
-        // D- comes from the workspace: a batch of PB pivots is requested before its steps, so
-        // the loads' latency is paid once per batch rather than once per step
-        constexpr int PB = 16;
-        for (int i0 = r; i0 < m - 1; i0 += PB) {
-            double dv[PB];
-#pragma unroll
-            for (int u = 0; u < PB; ++u) dv[u] = i0 + u < m - 1 ? Dm[(long long)(i0 + u + 1) * MP + j] : 1.0;
-#pragma unroll
-            for (int u = 0; u < PB; ++u) {
-                const int i = i0 + u;
-                if (i < m - 1) {
-                    const double zi = -(ee[i] * rcp_refined(dv[u])) * zp;
-                    Z[(i + 1) * LZ + j] = zi;
-                    nrm = fma(zi, zi, nrm);
-                    zp = zi;
-                }
+    // ---- 3. eigenvectors of the tridiagonal: twisted factorisation, two lanes per eigenvalue j:
+    // lane j (waves 0-1) runs the forward pivots D+ (LDS, Z's column j) and then the vector above
+    // the twist; lane 128 + j (waves 2-3) runs the backward pivots D- (workspace), the twist index
+    // and the vector below it. Every quantity has the one-lane form's expressions.
+    {
+        constexpr int BL = 128;                               // first lane of the backward half
+        constexpr int PB = 16;                                // D- loads requested together
+        const bool fw = tid < m, bw = tid >= BL && tid < BL + m;
+        const int j = fw ? tid : tid - BL;
+        int* rtw = cb;                                        // twist index per eigenvalue
+        double* nrs = xb;                                     // [2][BL]: the halves' squared norms
+        double sg = 0.0;
+        if (fw || bw) sg = 0.5 * (lo[j] + hi[j]);
+        if (fw) {
+            double dp = tri_guard(dd[0] - sg, pivmin);
+            Z[j] = dp;
+            for (int i = 1; i < m; ++i) {
+                dp = tri_guard(fma(-e2[i - 1], rcp_refined(dp), dd[i] - sg), pivmin);
+                Z[i * LZ + j] = dp;
             }
         }
-        Z[r * LZ + j] = 1.0;
-        const double inv = 1.0 / sqrt(nrm);
-        for (int i = 0; i < m; ++i) Z[i * LZ + j] = Z[i * LZ + j] * inv;
+        if (bw) {
+            double dm = tri_guard(dd[m - 1] - sg, pivmin);
+            Dm[(long long)(m - 1) * MP + j] = dm;
+            for (int i = m - 2; i >= 0; --i) {
+                dm = tri_guard(fma(-e2[i], rcp_refined(dm), dd[i] - sg), pivmin);
+                Dm[(long long)i * MP + j] = dm;
+            }
+        }
+        sync();
+        int r = 0;
+        if (bw) {
+            // r minimises |gamma_i| = |D+_i + D-_i - (d_i - sg)|, scanned from the bottom (ties: the
+            // lower row), the D- values read back in batches
+            double best = INFINITY;
+            for (int i0 = m - 1; i0 >= 0; i0 -= PB) {
+                double dv[PB];
+#pragma unroll
+                for (int u = 0; u < PB; ++u) dv[u] = i0 - u >= 0 ? Dm[(long long)(i0 - u) * MP + j] : 0.0;
+#pragma unroll
+                for (int u = 0; u < PB; ++u) {
+                    const int i = i0 - u;
+                    if (i >= 0) {
+                        const double g = fabs(Z[i * LZ + j] + dv[u] - (dd[i] - sg));
+                        if (i == m - 1 || g < best) { best = g; r = i; }
+                    }
+                }
+            }
+            rtw[j] = r;
+        }
+        sync();
+        double nrm = 0.0;
+        if (fw) {
+            r = rtw[j];
+            double zp = 1.0;
+            for (int i = r - 1; i >= 0; --i) {
+                const double zi = -(ee[i] * rcp_refined(Z[i * LZ + j])) * zp;
+                Z[i * LZ + j] = zi;
+                nrm = fma(zi, zi, nrm);
+                zp = zi;
+            }
+            Z[r * LZ + j] = 1.0;
+            nrs[j] = nrm;
+        }
+        if (bw) {
+            double zp = 1.0;
+            for (int i0 = r; i0 < m - 1; i0 += PB) {
+                double dv[PB];
+#pragma unroll
+                for (int u = 0; u < PB; ++u) dv[u] = i0 + u < m - 1 ? Dm[(long long)(i0 + u + 1) * MP + j] : 1.0;
+#pragma unroll
+                for (int u = 0; u < PB; ++u) {
+                    const int i = i0 + u;
+                    if (i < m - 1) {
+                        const double zi = -(ee[i] * rcp_refined(dv[u])) * zp;
+                        Z[(i + 1) * LZ + j] = zi;
+                        nrm = fma(zi, zi, nrm);
+                        zp = zi;
+                    }
+                }
+            }
+            nrs[BL + j] = nrm;
+        }
+        sync();
+        if (fw || bw) {
+            const double inv = 1.0 / sqrt((1.0 + nrs[j]) + nrs[BL + j]);
+            const int i_lo = fw ? 0 : r + 1, i_hi = fw ? r + 1 : m;     // rows [i_lo, i_hi)
+            for (int i = i_lo; i < i_hi; ++i) Z[i * LZ + j] = Z[i * LZ + j] * inv;
+        }
     }
     sync();
 

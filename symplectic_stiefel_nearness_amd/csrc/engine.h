@@ -36,6 +36,12 @@
 #ifndef SSN_CG_PREFETCH
 #define SSN_CG_PREFETCH 3
 #endif
+#ifndef SSN_QUAD_SCHED_BARRIER
+#define SSN_QUAD_SCHED_BARRIER 1
+#endif
+#ifndef SSN_QUAD_PREFETCH
+#define SSN_QUAD_PREFETCH 2     // k-steps of operand fragments in flight in the quadrant products
+#endif
 #ifndef SSN_CG_SCHED_BARRIER
 #define SSN_CG_SCHED_BARRIER 1
 #endif
@@ -1685,7 +1691,7 @@ template <int NT, class BF>
 __device__ __forceinline__ void quad_mma(__amdgpu_buffer_rsrc_t rA, int vA, double bsign, BF bfrag,
                                          d4 (&acc)[QuadCfg<NT>::H][QuadCfg<NT>::H]) {
     constexpr int MP = QuadCfg<NT>::MP, H = QuadCfg<NT>::H, K4 = QuadCfg<NT>::K4;
-    constexpr int D = 2;
+    constexpr int D = SSN_QUAD_PREFETCH;
     double ar[D][H];
 #pragma unroll
     for (int d = 0; d < D; ++d)
@@ -1704,7 +1710,7 @@ __device__ __forceinline__ void quad_mma(__amdgpu_buffer_rsrc_t rA, int vA, doub
 #pragma unroll
             for (int i = 0; i < H; ++i) ar[kb % D][i] = buf_ld(rA, vA, (4 * (kb + D) * MP + 16 * i) * 8);
         }
-        __builtin_amdgcn_sched_barrier(0);
+        if (SSN_QUAD_SCHED_BARRIER) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -1714,7 +1720,7 @@ template <int NT>
 __device__ __forceinline__ void quad_mma_gg(__amdgpu_buffer_rsrc_t rA, int vA, __amdgpu_buffer_rsrc_t rB, int vB,
                                             double bsign, d4 (&acc)[QuadCfg<NT>::H][QuadCfg<NT>::H]) {
     constexpr int MP = QuadCfg<NT>::MP, H = QuadCfg<NT>::H, K4 = QuadCfg<NT>::K4;
-    constexpr int D = 2;
+    constexpr int D = SSN_QUAD_PREFETCH;
     double ar[D][H], br[D][H];
 #pragma unroll
     for (int d = 0; d < D; ++d)
@@ -1739,7 +1745,7 @@ __device__ __forceinline__ void quad_mma_gg(__amdgpu_buffer_rsrc_t rA, int vA, _
                 br[kb % D][i] = buf_ld(rB, vB, (4 * (kb + D) * MP + 16 * i) * 8);
             }
         }
-        __builtin_amdgcn_sched_barrier(0);
+        if (SSN_QUAD_SCHED_BARRIER) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
