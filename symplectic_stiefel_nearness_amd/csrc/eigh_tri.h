@@ -36,6 +36,7 @@ namespace ssn {
 
 constexpr double TRI_SAFMIN = 2.2250738585072014e-308;
 constexpr double TRI_ORTHO_TOL = 1e-8;    // largest |V^T V - I| entry the Newton-Schulz step accepts
+constexpr int TRI_RESCALE = 8;            // Sturm-recurrence steps between power-of-two rescalings
 constexpr int TRI_MAX_ROUNDS = 128;       // multisection rounds (about 21 are needed at m = 96)
 constexpr int EX_TRI_TICKS = 10;          // extra slots 10..14: device ticks at the end of each phase
 #ifndef SSN_TRI_WAVES
@@ -80,15 +81,19 @@ __device__ __forceinline__ double rcp_refined(double x) {
 // sequence p_0 = 1, p_1 = d_0 - x, p_{i+1} = (d_i - x) p_i - e_{i-1}^2 p_{i-1} (the leading
 // principal minors of T - x I; a change at i is dstebz's negative pivot q_i = p_i / p_{i-1}).
 // One multiply-add per step on the dependence chain, no division; the pair (p_i, p_{i-1}) is
-// rescaled by a power of two every 4 steps (exact, signs kept). An exact zero counts as positive.
+// rescaled by a power of two every TRI_RESCALE steps (exact, signs kept: the recurrence is
+// homogeneous, so the counts do not depend on the period). Between rescalings the pair's larger
+// magnitude grows by at most a factor 2 ||T|| + ||T||^2 per step and shrinks by at most about
+// e_min^2 / (1 + 2 ||T||) per two steps (no split: e_min > 64 eps ||T||), so 8 steps stay far
+// inside the double range. An exact zero counts as positive.
 __device__ __forceinline__ int tri_count(const double* d, const double* e2, int m, double x) {
     double p1 = d[0] - x, p0 = 1.0;
     bool neg = p1 < 0.0;
     int cnt = neg ? 1 : 0;
     int i = 1;
-    for (; i + 4 <= m; i += 4) {
+    for (; i + TRI_RESCALE <= m; i += TRI_RESCALE) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < TRI_RESCALE; ++u) {
             const double pn = fma(d[i + u] - x, p1, -(e2[i + u - 1] * p0));
             p0 = p1; p1 = pn;
             const bool ng = pn < 0.0;
