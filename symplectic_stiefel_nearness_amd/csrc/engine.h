@@ -699,9 +699,11 @@ __device__ int jacobi(const Ctx& c, double* J, double* V, int LV, int vdummy, in
 __device__ __forceinline__ int bidx(int I, int J) { return I * (I + 1) / 2 + J; }
 
 // One wave: Cholesky of the 16x16 SPD block B (lower part read), replaced in place by
-// the inverse X of its factor (lower, zero above). Lane i < 16 holds row i in registers;
-// values move across lanes with readlane only (to scalar registers: the engine has no vector
-// registers to spare for broadcast copies). Each pivot takes one reciprocal square root
+// the inverse X of its factor (lower, zero above). Lane i < 16 holds row i in registers.
+// During the factorisation values move across lanes with readlane (to scalar registers: the
+// engine has no vector registers to spare for broadcast copies there); for the inverse, L is
+// written back to B and read from it (broadcast reads), which takes the readlanes off the
+// inverse's dependent chain. Each pivot takes one reciprocal square root
 // (hardware estimate plus two Newton steps) instead of a square root and a divide, and the
 // inverse is right-looking, so the dependent chains stay short. The updates of a pivot run on
 // every lane: lanes i <= j only change entries above the diagonal, which nothing reads, and no
@@ -730,11 +732,17 @@ __device__ __forceinline__ bool diag_factor_invert(double* B, int lane) {
     double x[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) x[i] = (lane == i) ? 1.0 : 0.0;
+    // L to B (lane i writes row i), then L_ik read from B (one address per read: a broadcast);
+    // the wave's LDS operations complete in order, so no barrier is needed
+    if (lane < 16) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) B[lane * 16 + k] = row[k];
+    }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         x[k] = x[k] * rinv[k];
 #pragma unroll
-        for (int i = k + 1; i < 16; ++i) x[i] = x[i] - readlane(row[k], i) * x[k];
+        for (int i = k + 1; i < 16; ++i) x[i] = x[i] - B[i * 16 + k] * x[k];
     }
     if (lane < 16) {
 #pragma unroll
