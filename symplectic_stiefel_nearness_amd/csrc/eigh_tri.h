@@ -4,9 +4,10 @@
 //
 //  1. Householder reduction to tridiagonal form (LAPACK's dsytd2 reflectors, lower storage) with
 //     the matrix in the registers of the eight waves: wave w holds rows w + 8 s, lane l columns l
-//     and l + 64, so the reflector's entries at a wave's rows are readlane broadcasts, y = A v is a
-//     register product plus eight partials per column through LDS, and the rank-2 update needs no
-//     LDS round trip. Reflector k is formed by the wave holding row k; two barriers per reflector.
+//     and l + 64, so the reflector's and w's entries at a wave's rows are broadcast reads from LDS
+//     (the reflector's row, and a per-wave copy of w), y = A v is a register product plus eight
+//     partials per column through LDS, and the rank-2 update needs no cross-wave exchange.
+//     Reflector k is formed by the wave holding row k; two barriers per reflector.
 //  2. Eigenvalues of the tridiagonal (d, e) by Sturm-count multisection (sign changes of the
 //     three-term minor recurrence, rescaled), P probes per eigenvalue per round, every eigenvalue
 //     bracketed to 2 eps |lambda| + eps ||T||.
@@ -158,8 +159,8 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
     // ---- 1. Householder reduction, A in registers (the lower triangle of Cr is read, UPLO='L').
     // Waves 0..NW-1 hold A: wave w the rows i = w + NW s (s < RW), lane l the columns l and l + 64.
     // Row i of a wave is wave-uniform, so the reflector's and w's entries at the wave's rows are
-    // readlane broadcasts of the coalesced copies every lane holds at its own columns. Reflector k
-    // is formed by the wave holding row k.
+    // broadcast reads from LDS (one address per read). Reflector k is formed by the wave holding
+    // row k.
     constexpr int NW = SSN_TRI_WAVES;
     static_assert(NW == 4 || NW == 8, "rows are held by 4 or 8 waves");
     constexpr int RW = MP / NW, NC = MP > 64 ? 2 : 1;
@@ -236,7 +237,7 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
 #pragma unroll
             for (int s = 0; s < RW; ++s) {
                 const int i = wave + NW * s;
-                const double vi = readlane_d(i < 64 ? vc[0] : vc[NC - 1], i & 63);
+                const double vi = hk[i];          // the reflector is in LDS: a broadcast read
                 if constexpr (KEEP_V) vr[s] = vi;
 #pragma unroll
                 for (int cc = 0; cc < NC; ++cc) yp[cc] = fma(ar[s][cc], vi, yp[cc]);
@@ -267,10 +268,18 @@ __device__ bool eigh_tri_body(const EighArgs& a, int b)
             double wc[NC];
 #pragma unroll
             for (int cc = 0; cc < NC; ++cc) wc[cc] = pc[cc] - K * vc[cc];
+            // w to this wave's own LDS row (written and read by the wave alone: its LDS operations
+            // complete in order, no barrier), then broadcast reads at the wave's rows
+            double* wl = part + NW * MP + wave * MP;
+#pragma unroll
+            for (int cc = 0; cc < NC; ++cc) {
+                const int col = lane + 64 * cc;
+                if (col < MP) wl[col] = wc[cc];
+            }
 #pragma unroll
             for (int s = 0; s < RW; ++s) {
                 const int i = wave + NW * s;
-                const double wi = readlane_d(i < 64 ? wc[0] : wc[NC - 1], i & 63);
+                const double wi = wl[i];
                 double vi;
                 if constexpr (KEEP_V) vi = vr[s];
                 else vi = readlane_d(i < 64 ? vc[0] : vc[NC - 1], i & 63);
