@@ -1632,26 +1632,29 @@ __device__ __forceinline__ void trial_square_lower(const Ctx& c, double* lds) {
     const double* Tl = lds + Cfg<NT>::OFF_PANEL;
     double* blk = lds + Cfg<NT>::OFF_BLK;
     const int r = opaque(c.r), q = opaque(c.q);
+    // tiles t = wave + 8 s; a wave with fewer than MAXT tiles repeats the last tile (and does not
+    // store it), so the k loop has no branches
     int tI[TC::MAXT], tJ[TC::MAXT];
 #pragma unroll
     for (int s = 0; s < TC::MAXT; ++s) {
         const int t = c.wave + 8 * s;
-        int I = 0, rem = t < TC::NB ? t : 0;
+        int I = 0, rem = t < TC::NB ? t : TC::NB - 1;
         while (rem > I) { rem -= I + 1; ++I; }
         tI[s] = I; tJ[s] = rem;
     }
     d4 acc[TC::MAXT];
 #pragma unroll
     for (int s = 0; s < TC::MAXT; ++s) acc[s] = zero4();
-#pragma unroll 4
+    // trial is exactly skew, so the A fragment trial[16 I + r][4 kb + q] is read as
+    // -trial[4 kb + q][16 I + r]: a row segment like the B fragment's (no bank conflicts between
+    // the rows of a fragment), bitwise the same value
+#pragma unroll
     for (int kb = 0; kb < K4; ++kb) {
 #pragma unroll
         for (int s = 0; s < TC::MAXT; ++s) {
-            if (c.wave + 8 * s < TC::NB) {
-                const double a = Tl[(16 * tI[s] + r) * LS + 4 * kb + q];
-                const double b = Tl[(4 * kb + q) * LS + 16 * tJ[s] + r];
-                acc[s] = mfma(a, b, acc[s]);
-            }
+            const double a = -Tl[(4 * kb + q) * LS + 16 * tI[s] + r];
+            const double b = Tl[(4 * kb + q) * LS + 16 * tJ[s] + r];
+            acc[s] = mfma(a, b, acc[s]);
         }
     }
 #pragma unroll
