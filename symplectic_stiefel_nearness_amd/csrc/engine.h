@@ -1443,14 +1443,18 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
             rz = fma(rv, z, rz);
             rr0 = fma(rv, rv, rr0);
             M[i * LS + j] = z; M[j * LS + i] = -z;
-            if (Z0g) { Z0g[i * MP + j] = z; Z0g[j * MP + i] = -z; }
         }
     }
-    for (int t = tid; t < MP; t += DT) {
-        M[t * LS + t] = 0.0;
-        if (Z0g) Z0g[t * MP + t] = 0.0;
+    for (int t = tid; t < MP; t += DT) M[t * LS + t] = 0.0;
+    wg_sum2(c, rz, rr0);    // (its barrier also completes p in LDS)
+    // the preconditioned initial residual (full skew) copied row by row from p in LDS, so the
+    // global stores are coalesced (their completion overlaps the first products)
+    if (Z0g) {
+        for (int e = tid; e < MP * MP; e += DT) {
+            const int i = e / MP, j = e - (e / MP) * MP;
+            Z0g[e] = M[i * LS + j];
+        }
     }
-    wg_sum2(c, rz, rr0);
     rz *= 2.0; rr0 *= 2.0;
     const double thr = ADJ ? tol * sqrt(rr0) : tol;
     int nh = 0;
@@ -1598,6 +1602,7 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
         sync();
     }
     sync();
+    // the solution (full skew) through p's LDS area, then copied row by row (coalesced stores)
 #pragma unroll
     for (int s = 0; s < UPT; ++s) {
         const int u = tid + s * DT;
@@ -1605,10 +1610,15 @@ __device__ __forceinline__ int cg_chained(const Ctx& c, double* lds, INIT init,
             int i, j;
             upper_ij<MP>(u, i, j);
             const double x = Xl[u];
-            Xg[i * MP + j] = x; Xg[j * MP + i] = -x;
+            M[i * LS + j] = x; M[j * LS + i] = -x;
         }
     }
-    for (int t = tid; t < MP; t += DT) Xg[t * MP + t] = 0.0;
+    for (int t = tid; t < MP; t += DT) M[t * LS + t] = 0.0;
+    sync();
+    for (int e = tid; e < MP * MP; e += DT) {
+        const int i = e / MP, j = e - (e / MP) * MP;
+        Xg[e] = M[i * LS + j];
+    }
     sync();
     return nh;
 }
@@ -2305,10 +2315,14 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
                 gg = fma(g, g, gg);
             });
         } else if constexpr (QuadCfg<NT>::OK) {
-            // Q = -W P = W^T P (W exactly skew)
+            // Q = -W P = W^T P (W exactly skew); Q^T is staged in the sandwich's LDS area (free
+            // here; odd row stride) and then stored row by row, so both global stores are coalesced
+            double* Ql = lds + QuadCfg<NT>::OFF;
+            constexpr int LY = QuadCfg<NT>::LY;
             quad_gemm2<NT>(c, W, P, 1.0, nullptr, nullptr, 0.0, [&](int, int i, int j, double v) {
-                Q[i * MP + j] = v; QT[j * MP + i] = v;
+                Q[i * MP + j] = v; Ql[j * LY + i] = v;
             });
+            each<NT>(c, [&](int i, int j, int e) { QT[e] = Ql[i * LY + j]; });
             sandwich_quad<NT>(c, lds, Cie, P, QT, lam_s, m, Jse, Sr, Si, G, gg);
         } else if constexpr (NT == 8) {
             strip_q<NT>(c, W, P, Q, QT);
