@@ -2488,8 +2488,15 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
                 wg_sum2(c, tail, first);
                 const double change = first + tail;
                 if (change <= 1e-4 * step * slope) {
-                    each_loaded<NT, 2>(c, [&](int e, double (&v)[2]) { v[0] = T1[e]; v[1] = T2[e]; },
-                                       [&](int, int, int e, const double (&v)[2]) { W[e] = v[0]; P[e] = v[1]; });
+                    if constexpr (EXT) {
+                        // accept: the trial and P' become W and P by exchanging the slots' roles
+                        // (no copy); the slots are put back after the loop
+                        double* tw = W; W = T1; T1 = tw;
+                        double* tp = P; P = T2; T2 = tp;
+                    } else {
+                        each_loaded<NT, 2>(c, [&](int e, double (&v)[2]) { v[0] = T1[e]; v[1] = T2[e]; },
+                                           [&](int, int, int e, const double (&v)[2]) { W[e] = v[0]; P[e] = v[1]; });
+                    }
                     sync();
                     accepted = true;
                     break;
@@ -2502,6 +2509,16 @@ __device__ __forceinline__ void dual_body(const DualArgs& a, int b)
         if (ns == caps.max_newton - 1) status = ST_NEWTON_CAP;
     }
     sync();
+    if (EXT && W != slot(S_W)) {
+        // W and P in the trial slots after an odd number of exchanges: back to their own slots
+        // (the change of basis, the omega pass and the backward read them there)
+        double* const Ws = slot(S_W);
+        double* const Ps = slot(S_P);
+        each_loaded<NT, 2>(c, [&](int e, double (&v)[2]) { v[0] = W[e]; v[1] = P[e]; },
+                           [&](int, int, int e, const double (&v)[2]) { Ws[e] = v[0]; Ps[e] = v[1]; });
+        T1 = W; T2 = P; W = Ws; P = Ps;
+        sync();
+    }
 
     // 4. back to the original basis (:58-64): omega = skew(V W V^T), P, Q rotated
     if constexpr (!EXT) {
