@@ -1,6 +1,6 @@
 """GPU parity of the HIP Dual solver against the CPU FP64 oracle (restatement of the reference).
 
-Tolerances (BASELINE north star): X and omega within 1e-10 relative (Frobenius, per target)
+Tolerances: X and omega within 1e-10 relative (Frobenius, per target)
 of the oracle on the same seeded targets; acceptance r_F <= 1e-8, r_G <= 1e-9, gap <= 1e-8
 recomputed on the CPU; Gram and recovery within 1e-13 relative of NumPy FP64.
 """
